@@ -1,0 +1,165 @@
+"""Benchmark of the fused lattice-Boltzmann step on one MI355X.
+
+Default workload: D3Q19 BGK Taylor-Green vortex, 256^3 periodic, FP32
+populations, Re 1600, Ma 0.1, pre-streaming (pull) fused kernel.  One
+"step" is one fused stream + collide pass over the whole grid.
+
+Prints one JSON line with MLUPS (cells * steps / 1e6 / s), the roofline of
+the dominant kernel (algorithmic HBM bytes 2 Q sizeof(storage) per cell
+against 8 TB/s), and a CPU baseline: the torch CPU restatement of lettuce's
+step (oracle/lettuce_torch.py) timed on a bounded sample of the same
+workload on this host.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+# workloads: (stencil, collision, storage, resolution, Re, Ma)
+WORKLOADS = {
+    "tgv3d_d3q19_bgk_256_f32": ("D3Q19", "bgk", "f32", 256, 1600, 0.1),
+    "tgv3d_d3q19_bgk_256_f16s": ("D3Q19", "bgk", "f16_shifted", 256, 1600, 0.1),
+    "tgv3d_d3q19_bgk_256_bf16s": ("D3Q19", "bgk", "bf16_shifted", 256, 1600, 0.1),
+    "tgv3d_d3q27_kbc_256_f32": ("D3Q27", "kbc", "f32", 256, 1600, 0.1),
+    "tgv2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.1),
+}
+BYTES = {"f32": 4, "f64": 8, "f16_shifted": 2, "bf16_shifted": 2}
+
+
+def build_sim(lt, name):
+    stencil, coll, storage, res, re, ma = WORKLOADS[name]
+    S = {"D2Q9": lt.D2Q9, "D3Q19": lt.D3Q19, "D3Q27": lt.D3Q27}[stencil]()
+    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage)
+    flow = lt.TaylorGreenVortex(ctx, res, re, ma, S, initialize_fneq=False)
+    tau = flow.units.relaxation_parameter_lu
+    collision = {"bgk": lt.BGKCollision(tau), "kbc": lt.KBCCollision(),
+                 "trt": lt.TRTCollision(tau, 1.0)}[coll]
+    sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
+    return sim, S
+
+
+def time_gpu(sim, steps, warmup):
+    step = sim._collide_and_stream
+    step.run(warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    step.run(steps)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / 1e3, wall
+
+
+def cpu_baseline(name, budget_s=15.0):
+    """The oracle (torch CPU restatement of lettuce's step) on a bounded
+    sample: the same lattice, collision and FP32 arithmetic on a 96^3 (3D)
+    or 512x128 (2D) grid, as many steps as fit the budget."""
+    from oracle import lettuce_torch as O
+    stencil, coll, storage, res, re, ma = WORKLOADS[name]
+    S = {"D2Q9": O.D2Q9, "D3Q19": O.D3Q19, "D3Q27": O.D3Q27}[stencil]()
+    sample = [512, 128] if S.d == 2 else 96
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    flow = O.TaylorGreenVortex(O.Context("cpu", torch.float32), sample, re, ma, S)
+    tau = flow.units.relaxation_parameter_lu
+    c = {"bgk": O.BGKCollision(tau), "kbc": O.KBCCollision(),
+         "trt": O.TRTCollision(tau, 1.0)}[coll]
+    sim = O.Simulation(flow, c, [], O.StreamingStrategy.PRE_STREAMING)
+    sim(1)  # warm-up
+    cells = int(np.prod(flow.resolution))
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        sim(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 200:
+            break
+    return {"value": round(steps * cells / 1e6 / el, 3), "unit": "MLUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"{stencil} {coll.upper()} TGV {'x'.join(map(str, flow.resolution))} "
+                      f"FP32, {steps} steps after 1 warm-up, torch CPU "
+                      f"({torch.get_num_threads()} threads)"}
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="tgv3d_d3q19_bgk_256_f32",
+                    choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the slab decomposition over xGMI is described in DESIGN.md); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+
+    import lettuce_amd as lt
+    sim, S = build_sim(lt, args.workload)
+    stencil, coll, storage, res, re, ma = WORKLOADS[args.workload]
+    cells = int(np.prod(sim.flow.resolution))
+    gpu_s, wall_s = time_gpu(sim, args.steps, args.warmup)
+    mlups = args.steps * cells / 1e6 / gpu_s
+    bytes_per_cell = 2 * S.q * BYTES[storage]
+    launch_s = gpu_s / args.steps
+    achieved = bytes_per_cell * cells / launch_s / 1e9
+    traffic = load_traffic(args.workload)
+    line = {
+        "metric": "MLUPS",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(launch_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage in ("f32", "f16_shifted", "bf16_shifted") else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": stencil,
+                   "collision": coll.upper(), "storage": storage,
+                   "grid": sim.flow.resolution, "cells": cells,
+                   "streaming": "pre (pull), fused", "Re": re, "Ma": ma},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic,
+                     "bytes_per_cell": bytes_per_cell},
+        "wall_s": round(wall_s, 4),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+/*
+ * lettuce_amd C-ABI: the MI355X (gfx950) lattice-Boltzmann time step.
+ *
+ * This is the drop-in boundary for lettuce's native step.  lettuce binds its
+ * native path as one Python callable per simulation (`invoke(simulation)`,
+ * lettuce/cuda_native/_template.py:35-40) that calls one pybind11 function
+ * `lettuce(f, f_next, ncm, nsm, tau_inv, ...)` taking at::Tensor arguments
+ * (_template.py:47-53, :72-81) and then swaps `flow.f, flow.f_next`
+ * (_template.py:39).  Here the same job is done by plain-C entry points that
+ * take raw device pointers, sizes and a hipStream_t (passed as void*), so any
+ * FFI (ctypes, cffi, a C++ host) can bind them.  No torch types appear here.
+ *
+ * Conventions
+ *  - Population storage is SoA, `f[q][i0][i1][i2]`, the last index contiguous,
+ *    exactly lettuce's `flow.f` of shape [Q, X, Y(, Z)] (lettuce/_flow.py:92).
+ *    2D lattices use i0 == 0 (n[0] == 1) with (i1, i2) = (x, y).
+ *  - Every entry point returns 0 on success or a negative LBM_E_* code;
+ *    lbm_last_error() returns a thread-local message for the last failure.
+ *  - Nothing here allocates or frees caller memory; launches are
+ *    stream-ordered and nothing synchronises the device.
+ */
+#ifndef LETTUCE_AMD_H
+#define LETTUCE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* error codes */
+#define LBM_OK 0
+#define LBM_E_ARG (-1)      /* invalid argument (shape, stride, enum, NULL) */
+#define LBM_E_VERSION (-2)  /* args->abi_version != LBM_ABI_VERSION          */
+#define LBM_E_UNSUPPORTED (-3) /* combination not available natively         */
+#define LBM_E_HIP (-4)      /* HIP runtime error reported at launch           */
+
+/* lattices: lettuce/ext/_stencil/{d2q9,d3q19,d3q27}.py (same direction order) */
+enum lbm_stencil { LBM_D2Q9 = 0, LBM_D3Q19 = 1, LBM_D3Q27 = 2 };
+
+/* population storage.  The *_SHIFTED formats hold f_q - w_q * rho0_shift;
+ * all arithmetic is done in FP32 (FP64 for LBM_F64). */
+enum lbm_storage {
+  LBM_F32 = 0,
+  LBM_F64 = 1,
+  LBM_F16_SHIFTED = 2,
+  LBM_BF16_SHIFTED = 3
+};
+
+/* collision operators on cells of op LBM_OP_COLLIDE:
+ *  NONE    lettuce/ext/_collision/no_collision.py:10-11
+ *  BGK     lettuce/ext/_collision/bgk_collision.py:17-22
+ *  TRT     lettuce/ext/_collision/trt_collision.py:16-27
+ *  KBC     lettuce/ext/_collision/kbc_collision.py:96-160 (D2Q9, D3Q27)
+ *  LEARNED KBC decomposition whose higher-order rates come from a per-cell
+ *          MLP (no lettuce counterpart; nearest hook MRTCollision,
+ *          lettuce/ext/_collision/mrt_collision.py:8-35), D2Q9 only. */
+enum lbm_collision {
+  LBM_COLL_NONE = 0,
+  LBM_COLL_BGK = 1,
+  LBM_COLL_TRT = 2,
+  LBM_COLL_KBC = 3,
+  LBM_COLL_LEARNED = 4
+};
+
+/* streaming strategy, same bit values as lettuce's StreamingStrategy
+ * (lettuce/cuda_native/_default_code_gen.py:13-25):
+ * bit 1 = stream before collision (pull), bit 0 = stream after (push). */
+enum lbm_streaming {
+  LBM_NO_STREAMING = 0,
+  LBM_POST_STREAMING = 1,
+  LBM_PRE_STREAMING = 2,
+  LBM_DOUBLE_STREAMING = 3
+};
+
+/* what a cell does in place of collision (one op per cell, the transformer
+ * selected by lettuce's no_collision_mask, lettuce/_simulation.py:258-305) */
+enum lbm_cell_op {
+  LBM_OP_COLLIDE = 0,     /* the simulation's collision                     */
+  LBM_OP_BOUNCE_BACK = 1, /* f_q <- f_opp(q)      bounce_back_boundary.py:17 */
+  LBM_OP_EQUILIBRIUM = 2, /* f <- feq(rho, u)     equilibrium_boundary_pu.py:79-84 */
+  LBM_OP_OUTLET = 3,      /* f <- feq(rho, u(x - dir)) equilibrium_outlet_p.py:63-73 */
+  LBM_OP_KEEP = 4         /* f unchanged (no transformer, halfway-BB solid) */
+};
+
+/* cell-class flags */
+#define LBM_CLASS_SOLID 0x01      /* halfway bounce-back solid cell          */
+#define LBM_CLASS_NEAR_SOLID 0x02 /* some neighbour x - e_q is SOLID         */
+#define LBM_CLASS_BC_FIELD 0x04   /* EQUILIBRIUM: rho,u read per cell from bc_field */
+
+/* One entry of the cell-class table (48 bytes).  A class is one distinct
+ * (no_collision_mask value, no_streaming_mask column, solid-neighbour) triple. */
+typedef struct lbm_cell_class {
+  uint8_t op;         /* enum lbm_cell_op                                      */
+  uint8_t flags;      /* LBM_CLASS_*                                           */
+  int8_t dir[3];      /* OUTLET: the velocity is taken at x - dir (i0,i1,i2)   */
+  uint8_t pad[3];
+  uint32_t no_stream; /* bit q set: population q is not streamed into the cell */
+  uint32_t pad2;
+  double rho;         /* EQUILIBRIUM / OUTLET density (lattice units)          */
+  double u[3];        /* EQUILIBRIUM velocity, lattice components (x, y, z)    */
+} lbm_cell_class;
+
+/* learned-collision parameters (device pointers); LBM_COLL_LEARNED only */
+typedef struct lbm_learned {
+  const uint16_t* w1; /* bf16 [32][6]   first layer, row-major (out, in)      */
+  const float* b1;    /* [32]                                                  */
+  const uint16_t* w2; /* bf16 [32][32]                                         */
+  const float* b2;    /* [32]                                                  */
+  const uint16_t* w3; /* bf16 [3][32]                                          */
+  const float* b3;    /* [3]                                                   */
+  float in_scale;     /* inputs are non-equilibrium moments / rho * in_scale   */
+  float rate_lo;      /* predicted rates are clamped into [rate_lo, rate_hi]   */
+  float rate_hi;
+  float stable_lo;    /* a raw rate outside [stable_lo, stable_hi] (or NaN)    */
+  float stable_hi;    /*   makes the cell fall back to KBC                     */
+  int32_t pad;
+} lbm_learned;
+
+typedef struct lbm_step_args {
+  int32_t abi_version; /* LBM_ABI_VERSION                                     */
+  int32_t stencil;     /* enum lbm_stencil                                    */
+  int32_t storage;     /* enum lbm_storage                                    */
+  int32_t collision;   /* enum lbm_collision                                  */
+  int32_t streaming;   /* enum lbm_streaming                                  */
+  int32_t n_classes;   /* entries in `classes` (0 when cell_class == NULL)    */
+  int64_t n[3];        /* extents (i0, i1, i2); 2D: n[0] == 1                 */
+  int64_t stride[4];   /* element strides of q, i0, i1, i2; stride[3] == 1    */
+  const void* f_in;    /* populations before the step                         */
+  void* f_out;         /* populations after the step (must not alias f_in)    */
+  const uint8_t* cell_class;      /* [n0*n1*n2] class ids, or NULL (all collide, periodic) */
+  const lbm_cell_class* classes;  /* device table of n_classes entries        */
+  const double* bc_field; /* [4][cells] (rho, ux, uy, uz) for LBM_CLASS_BC_FIELD, else NULL */
+  double tau;          /* BGK / TRT tau_plus / KBC tau                        */
+  double tau_minus;    /* TRT tau_minus                                       */
+  double rho0_shift;   /* shifted storage reference density (usually 1)       */
+  const lbm_learned* learned; /* host pointer to parameters, LEARNED only      */
+  int64_t fallback_count_ptr_reserved; /* must be 0                           */
+  double reserved[3];
+} lbm_step_args;
+
+/* ABI version of the loaded library (LBM_ABI_VERSION it was built with). */
+int lbm_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* lbm_last_error(void);
+
+/* sizeof(lbm_step_args), sizeof(lbm_cell_class), sizeof(lbm_learned): lets a
+ * binding check its struct mirror. */
+int64_t lbm_struct_size(int which);
+
+/* One time step: f_out = step(f_in).  Replaces the generated
+ * `lettuce(f, f_next, ...)` kernel host function
+ * (lettuce/cuda_native/_template.py:72-81, launched by invoke() at :35-40);
+ * the caller swaps f_in / f_out afterwards as invoke() does (:39). */
+int lbm_step(const lbm_step_args* args, void* stream);
+
+/* `nsteps` time steps, alternating f_in and f_out (step k reads the buffer
+ * written by step k-1).  The result is in f_out when nsteps is odd and in
+ * f_in when it is even; f_in is overwritten when nsteps >= 2.  Replaces the
+ * per-step Python loop of Simulation.__call__ (lettuce/_simulation.py:317-320)
+ * when no reporter runs in between. */
+int lbm_run(const lbm_step_args* args, int64_t nsteps, void* stream);
+
+/* f[q][cell] = feq_q(rho[cell], u[:, cell]) written in the given storage:
+ * QuadraticEquilibrium (lettuce/ext/_equilibrium/quadratic_equilibrium.py:11-24)
+ * as used by Flow.initialize (lettuce/_flow.py:127-143).
+ * rho: [cells], u: [D][cells] in FP32 (storage F32/F16/BF16) or FP64 (F64),
+ * contiguous; f uses `stride` as in lbm_step_args. */
+int lbm_equilibrium(int32_t stencil, int32_t storage, const int64_t n[3],
+                    const int64_t stride[4], const void* rho, const void* u,
+                    double rho0_shift, void* f, void* stream);
+
+/* Decode any storage into FP32 or FP64 populations (out_f64 selects), i.e.
+ * the true f_q = stored_q + w_q * rho0_shift for the shifted formats.
+ * `out` is contiguous [Q][cells]. */
+int lbm_decode(int32_t stencil, int32_t storage, const int64_t n[3],
+               const int64_t stride[4], const void* f, double rho0_shift,
+               void* out, int32_t out_f64, void* stream);
+
+/* Encode contiguous FP32/FP64 populations `in` ([Q][cells]; in_f64 selects)
+ * into `f` in the given storage (inverse of lbm_decode). */
+int lbm_encode(int32_t stencil, int32_t storage, const int64_t n[3],
+               const int64_t stride[4], const void* in, int32_t in_f64,
+               double rho0_shift, void* f, void* stream);
+
+/* Moments in FP64: rho[cell] = sum_q f_q, u[d][cell] = j_d / rho
+ * (Flow.rho / Flow.j / Flow.u, lettuce/_flow.py:157-193).  rho and u are
+ * contiguous FP64 arrays of [cells] and [D][cells]. */
+int lbm_moments_f64(int32_t stencil, int32_t storage, const int64_t n[3],
+                    const int64_t stride[4], const void* f, double rho0_shift,
+                    double* rho, double* u, void* stream);
+
+/* FP64 global reductions over the cells whose `mask` byte is 0 (mask may be
+ * NULL = all cells).  Writes out[0..4] = { sum rho, sum j_x, sum j_y, sum j_z,
+ * sum |u|^2 / 2 }.  The kinetic-energy sum is IncompressibleKineticEnergy in
+ * lattice units before the PU conversion (observable_reporter.py:34-42).
+ * `workspace` holds at least lbm_reduce_workspace_bytes(n) bytes. */
+int64_t lbm_reduce_workspace_bytes(const int64_t n[3]);
+int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3],
+                   const int64_t stride[4], const void* f, double rho0_shift,
+                   const uint8_t* mask, double* out, void* workspace,
+                   void* stream);
+
+/* FP64 enstrophy sum, sum over cells of |curl u|^2 with u in lattice units
+ * and 6th-order periodic central differences (dx = 1), the sum computed by
+ * Enstrophy (observable_reporter.py:45-68) via torch_gradient
+ * (lettuce/util/utility.py:37-99) before the PU scaling.  u is FP64
+ * [D][cells] as written by lbm_moments_f64. */
+int lbm_enstrophy_f64(int32_t stencil, const int64_t n[3], const double* u,
+                      double* out, void* workspace, void* stream);
+
+/* Momentum-exchange force on halfway-bounce-back solids from the stored
+ * post-collision populations: F = 2 sum_links f_d(x) e_d over links
+ * (x fluid, x + e_d solid), the force of
+ * examples/advanced_projects/efficient_bounce_back_obstacle/boundary/
+ * halfway_bounce_back_boundary.py:211-218.  links: int32 [L][2] = (cell, d).
+ * out[0..2] FP64 force components. */
+int lbm_link_force_f64(int32_t stencil, int32_t storage, const int64_t n[3],
+                       const int64_t stride[4], const void* f,
+                       double rho0_shift, const int32_t* links, int64_t n_links,
+                       double* out, void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LETTUCE_AMD_H */

@@ -1,0 +1,124 @@
+"""Execution context: device, arithmetic dtype and population storage.
+
+Mirrors lettuce's ``Context`` (lettuce/_context.py:9-122): the same
+constructor arguments, defaults and tensor factories.  Differences:
+
+* the step always runs through the HIP kernels, so a context that is to run
+  a simulation must live on a ROCm device (``use_native`` may not be False
+  there);
+* ``storage`` selects how populations are held in HBM: ``"f32"``/``"f64"``
+  (the dtype itself) or ``"f16_shifted"``/``"bf16_shifted"`` (f_q - w_q rho0
+  in 16 bits, FP32 arithmetic).  lettuce has no such option.
+"""
+from typing import List, Optional, Union
+
+import numpy as np
+import torch
+
+__all__ = ["Context"]
+
+_STORAGE_DTYPES = {"f32": torch.float32, "f64": torch.float64,
+                   "f16_shifted": torch.float16,
+                   "bf16_shifted": torch.bfloat16}
+
+
+class Context:
+    def __init__(self, device: Optional[Union[torch.device, str]] = None,
+                 dtype: Optional[torch.dtype] = None,
+                 use_native: Optional[bool] = None,
+                 storage: Optional[str] = None):
+        cuda = torch.cuda.is_available()
+        if device is None:
+            if use_native is None:
+                use_native = cuda
+            device = torch.device("cuda:0" if cuda else "cpu")
+        device = torch.device(device)
+        if device.type not in ("cuda", "cpu"):
+            raise ValueError(f"lettuce_amd runs on ROCm GPUs (torch 'cuda' "
+                             f"devices); {device} is not supported")
+        if device.type == "cuda":
+            if not cuda:
+                raise RuntimeError("a ROCm device was requested but none is "
+                                   "available")
+            if use_native is False:
+                raise ValueError(
+                    "lettuce_amd has no torch step: use_native=False is not "
+                    "available on a GPU device")
+            use_native = True
+        else:
+            if use_native:
+                raise ValueError("the native step needs a ROCm device")
+            use_native = False
+
+        dtype = dtype or torch.float32
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError(
+                f"arithmetic dtype must be float32 or float64, got {dtype}; "
+                f"16-bit populations are selected with storage="
+                f"'f16_shifted' or 'bf16_shifted' (FP32 arithmetic)")
+        if storage is None:
+            storage = "f64" if dtype == torch.float64 else "f32"
+        if storage not in _STORAGE_DTYPES:
+            raise ValueError(f"unknown storage {storage!r}; expected one of "
+                             f"{sorted(_STORAGE_DTYPES)}")
+        if dtype == torch.float64 and storage != "f64":
+            raise ValueError("float64 arithmetic needs storage='f64'")
+        if dtype == torch.float32 and storage == "f64":
+            raise ValueError("storage='f64' needs dtype=torch.float64")
+
+        self.device = device
+        self.dtype = dtype
+        self.use_native = use_native
+        self.storage = storage
+
+    @property
+    def storage_dtype(self) -> torch.dtype:
+        return _STORAGE_DTYPES[self.storage]
+
+    @property
+    def shifted(self) -> bool:
+        return self.storage.endswith("_shifted")
+
+    def synchronize(self):
+        if self.device.type == "cuda":
+            torch.cuda.synchronize(self.device)
+
+    def empty_tensor(self, size: Union[List[int], torch.Size], *args,
+                     dtype=None, **kwargs) -> torch.Tensor:
+        return torch.empty(size, *args, **kwargs, device=self.device,
+                           dtype=(dtype or self.dtype))
+
+    def zero_tensor(self, size, *args, dtype=None, **kwargs) -> torch.Tensor:
+        return torch.zeros(size, *args, **kwargs, device=self.device,
+                           dtype=(dtype or self.dtype))
+
+    def full_tensor(self, size, value, *args, dtype=None, **kwargs
+                    ) -> torch.Tensor:
+        return torch.full(size, value, *args, **kwargs, device=self.device,
+                          dtype=(dtype or self.dtype))
+
+    def one_tensor(self, size, *args, dtype=None, **kwargs) -> torch.Tensor:
+        return torch.ones(size, *args, **kwargs, device=self.device,
+                          dtype=(dtype or self.dtype))
+
+    def convert_to_tensor(self, array, *args,
+                          dtype: Optional[torch.dtype] = None, **kwargs
+                          ) -> torch.Tensor:
+        if dtype is None:
+            bool_like = (bool, torch.bool, torch.uint8, np.uint8,
+                         np.dtype(bool), np.dtype(np.uint8))
+            if hasattr(array, "dtype") and array.dtype in bool_like:
+                dtype = torch.uint8
+            else:
+                dtype = self.dtype
+        if isinstance(array, torch.Tensor):
+            return array.to(*args, **kwargs, device=self.device, dtype=dtype)
+        array = np.ascontiguousarray(np.array(array))
+        return torch.from_numpy(array).to(*args, **kwargs, device=self.device,
+                                          dtype=dtype)
+
+    @staticmethod
+    def convert_to_ndarray(tensor) -> np.ndarray:
+        if isinstance(tensor, torch.Tensor):
+            return tensor.detach().cpu().numpy()
+        return np.array(tensor)

@@ -1,0 +1,219 @@
+"""Flow: the simulation state and its plugin interfaces.
+
+Mirrors lettuce's ``Equilibrium``/``Boundary``/``Flow`` (lettuce/_flow.py:17-
+268).  The populations live in HBM in the context's storage format
+(``context.storage``); ``flow.f`` always presents the true populations in
+the arithmetic dtype.  For FP32/FP64 storage ``flow.f`` *is* the state
+tensor (no copy), exactly like lettuce; for shifted 16-bit storage it is a
+decoded copy and assigning ``flow.f`` re-encodes.
+
+Moments (rho, j, u) are computed by the native FP64 moments kernel.
+"""
+import warnings
+from abc import ABC, abstractmethod
+from typing import List, Optional, Union
+
+import numpy as np
+import torch
+
+from ._stencil import TorchStencil
+from . import _ops
+
+__all__ = ["Equilibrium", "Boundary", "Flow"]
+
+
+class Equilibrium(ABC):
+    @abstractmethod
+    def __call__(self, flow: "Flow", rho=None, u=None) -> torch.Tensor:
+        ...
+
+    @abstractmethod
+    def native_available(self) -> bool:
+        ...
+
+    @abstractmethod
+    def native_generator(self):
+        ...
+
+
+class Boundary(ABC):
+    @abstractmethod
+    def __call__(self, flow: "Flow"):
+        ...
+
+    @abstractmethod
+    def make_no_collision_mask(self, shape: List[int], context
+                               ) -> Optional[torch.Tensor]:
+        ...
+
+    @abstractmethod
+    def make_no_streaming_mask(self, shape: List[int], context
+                               ) -> Optional[torch.Tensor]:
+        ...
+
+    @abstractmethod
+    def native_available(self) -> bool:
+        ...
+
+    @abstractmethod
+    def native_generator(self, index: int):
+        ...
+
+
+class Flow(ABC):
+    initialize_pressure: bool = False
+    initialize_fneq: bool = False
+
+    def __init__(self, context, resolution: List[int], units, stencil,
+                 equilibrium):
+        self.context = context
+        self.resolution = list(resolution)
+        self.units = units
+        self.stencil = stencil
+        self.torch_stencil = TorchStencil(stencil, context)
+        self.equilibrium = equilibrium
+        self.i = 0
+        self._state = context.empty_tensor([stencil.q, *self.resolution],
+                                           dtype=context.storage_dtype)
+        self._state_next = None
+        self.initialize()
+
+    # -- boundaries ---------------------------------------------------------
+    @property
+    def pre_boundaries(self) -> List["Boundary"]:
+        return []
+
+    @property
+    def post_boundaries(self) -> List["Boundary"]:
+        warnings.warn(
+            "post_boundaries is not defined by the flow class; falling back "
+            "to the deprecated `boundaries` property", stacklevel=2)
+        return self.boundaries
+
+    @property
+    def boundaries(self) -> List["Boundary"]:
+        return []
+
+    @property
+    def post_streaming_boundaries(self) -> List["Boundary"]:
+        """Boundaries applied after streaming (halfway bounce-back), as in
+        EbbSimulation (examples/.../simulation/ebb_simulation.py:21-25)."""
+        return []
+
+    # -- initial state ------------------------------------------------------
+    @abstractmethod
+    def initial_pu(self):
+        """initial (pressure, velocity) in physical units"""
+        ...
+
+    def initialize(self):
+        """Equilibrium initialisation (lettuce/_flow.py:127-143)."""
+        p, u = self.initial_pu()
+        rho = self.context.convert_to_tensor(
+            self.units.convert_pressure_pu_to_density_lu(p))
+        u = self.context.convert_to_tensor(self.units.convert_velocity_to_lu(u))
+        if self.initialize_pressure:
+            from .util import pressure_poisson
+            rho = pressure_poisson(self.units, u, rho)
+        self.f = self.equilibrium(self, rho=rho, u=u)
+        if self.initialize_fneq:
+            from .util import initialize_f_neq
+            self.f = initialize_f_neq(self)
+
+    # -- populations --------------------------------------------------------
+    @property
+    def rho0_shift(self) -> float:
+        return 1.0
+
+    @property
+    def f(self) -> torch.Tensor:
+        if self.context.shifted:
+            return _ops.decode(self.stencil, self._state, self.rho0_shift,
+                               self.context.dtype)
+        return self._state
+
+    @f.setter
+    def f(self, value: torch.Tensor):
+        value = self.context.convert_to_tensor(value)
+        if tuple(value.shape) != tuple(self._state.shape):
+            raise ValueError(f"f must have shape {tuple(self._state.shape)}, "
+                             f"got {tuple(value.shape)}")
+        if self.context.shifted:
+            _ops.encode(self.stencil, value, self._state, self.rho0_shift)
+        else:
+            self._state = value.contiguous()
+
+    @property
+    def f_next(self) -> torch.Tensor:
+        """The second (B) buffer, created on first use
+        (lettuce/_flow.py:145-151); in the storage format."""
+        if self._state_next is None or self._state_next.shape != self._state.shape:
+            self._state_next = torch.empty_like(self._state)
+        return self._state_next
+
+    @f_next.setter
+    def f_next(self, value: torch.Tensor):
+        self._state_next = value
+
+    def swap(self):
+        """f, f_next = f_next, f (lettuce/cuda_native/_template.py:39)."""
+        self._state, self._state_next = self.f_next, self._state
+
+    # -- moments ------------------------------------------------------------
+    def _moments(self, f=None):
+        if f is None:
+            return _ops.moments_f64(self.stencil, self._state, self.rho0_shift)
+        f = self.context.convert_to_tensor(f).contiguous()
+        return _ops.moments_f64(self.stencil, f)
+
+    def rho(self, f: Optional[torch.Tensor] = None) -> torch.Tensor:
+        rho, _ = self._moments(f)
+        return rho[None, ...].to(self.context.dtype)
+
+    @property
+    def rho_pu(self) -> torch.Tensor:
+        return self.units.convert_density_to_pu(self.rho())
+
+    @property
+    def p_pu(self) -> torch.Tensor:
+        return self.units.convert_density_lu_to_pressure_pu(self.rho())
+
+    @property
+    def u_pu(self):
+        return self.units.convert_velocity_to_pu(self.u())
+
+    def j(self, f: Optional[torch.Tensor] = None) -> torch.Tensor:
+        rho, u = self._moments(f)
+        return (u * rho[None, ...]).to(self.context.dtype)
+
+    def u(self, f: Optional[torch.Tensor] = None, rho=None,
+          acceleration=None) -> torch.Tensor:
+        if acceleration is not None:
+            raise NotImplementedError("forcing is not supported natively")
+        if rho is None:
+            _, u = self._moments(f)
+            return u.to(self.context.dtype)
+        return self.j(f) / rho
+
+    @property
+    def velocity(self):
+        return self.u()
+
+    def incompressible_energy(self, f: Optional[torch.Tensor] = None
+                              ) -> torch.Tensor:
+        u = self.u(f)
+        return 0.5 * (u * u).sum(dim=0)
+
+    # -- checkpoint ---------------------------------------------------------
+    def dump(self, filename):
+        """Save the true populations (lettuce/_flow.py:258-261) as a .npy
+        file (no pickle)."""
+        with open(filename, "wb") as fh:
+            np.save(fh, self.context.convert_to_ndarray(self.f),
+                    allow_pickle=False)
+
+    def load(self, filename):
+        with open(filename, "rb") as fh:
+            self.f = self.context.convert_to_tensor(
+                np.load(fh, allow_pickle=False), dtype=self.context.dtype)
+        self._state_next = None

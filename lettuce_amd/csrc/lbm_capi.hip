@@ -1,0 +1,628 @@
+// extern "C" entry points of liblettuce_amd.so (declared in
+// include/lettuce_amd.h) and the FP64 utility kernels: equilibrium
+// initialisation, storage encode/decode, moments, global reductions,
+// enstrophy and the halfway bounce-back link force.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <type_traits>
+
+#include "lbm_step_impl.h"
+
+namespace lbm {
+KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode);
+KernelFn pick_kernel_d3q19(int storage, int coll, bool mask, int mode);
+KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
+}  // namespace lbm
+
+using namespace lbm;
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LBM_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  return LBM_OK;
+}
+
+static int stencil_q(int stencil) {
+  switch (stencil) {
+    case LBM_D2Q9: return 9;
+    case LBM_D3Q19: return 19;
+    case LBM_D3Q27: return 27;
+  }
+  return -1;
+}
+static int stencil_d(int stencil) { return stencil == LBM_D2Q9 ? 2 : 3; }
+
+static int storage_bytes(int storage) {
+  switch (storage) {
+    case LBM_F32: return 4;
+    case LBM_F64: return 8;
+    case LBM_F16_SHIFTED:
+    case LBM_BF16_SHIFTED: return 2;
+  }
+  return -1;
+}
+
+// Shape / stride validation shared by every entry point.  The population
+// array is [Q][n0][n1][n2] with unit stride along i2, rows of at least n2
+// elements and no overlap between populations.
+static int check_layout(int stencil, int storage, const int64_t n[3], const int64_t stride[4]) {
+  const int Q = stencil_q(stencil);
+  if (Q < 0) return fail(LBM_E_ARG, "unknown stencil %d", stencil);
+  if (storage_bytes(storage) < 0) return fail(LBM_E_ARG, "unknown storage %d", storage);
+  if (!n || !stride) return fail(LBM_E_ARG, "NULL extents or strides");
+  if (n[0] < 1 || n[1] < 1 || n[2] < 1)
+    return fail(LBM_E_ARG, "extents must be positive, got (%lld, %lld, %lld)",
+                (long long)n[0], (long long)n[1], (long long)n[2]);
+  if (stencil_d(stencil) == 2 && n[0] != 1)
+    return fail(LBM_E_ARG, "2D lattices need n[0] == 1, got %lld", (long long)n[0]);
+  if (stride[3] != 1) return fail(LBM_E_ARG, "the last axis must be contiguous (stride 1)");
+  if (stride[2] < n[2] || (n[0] > 1 && stride[1] < stride[2] * n[1]))
+    return fail(LBM_E_ARG, "overlapping spatial strides");
+  const int64_t span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
+  if (stride[0] < span) return fail(LBM_E_ARG, "population stride %lld smaller than one population (%lld)",
+                                    (long long)stride[0], (long long)span);
+  // all cell offsets are 32-bit inside the kernels
+  if (span > (int64_t)0x7fffffff) return fail(LBM_E_ARG, "grid too large for 32-bit cell offsets");
+  if ((int64_t)Q * stride[0] > ((int64_t)1 << 40)) return fail(LBM_E_ARG, "population array too large");
+  return LBM_OK;
+}
+
+static int block_size_for(int64_t n2) {
+  if (n2 % 256 == 0) return 256;
+  if (n2 % 192 == 0) return 192;
+  if (n2 % 128 == 0) return 128;
+  if (n2 <= 64) return 64;
+  if (n2 <= 128) return 128;
+  return 256;
+}
+
+static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
+  if (!a) return fail(LBM_E_ARG, "NULL args");
+  if (a->abi_version != LBM_ABI_VERSION)
+    return fail(LBM_E_VERSION, "abi_version %d != library %d", a->abi_version, LBM_ABI_VERSION);
+  int rc = check_layout(a->stencil, a->storage, a->n, a->stride);
+  if (rc) return rc;
+  if (!a->f_in || !a->f_out) return fail(LBM_E_ARG, "NULL population pointer");
+  if (a->f_in == a->f_out) return fail(LBM_E_ARG, "f_in and f_out must be distinct buffers");
+  if (a->streaming < 0 || a->streaming > 3) return fail(LBM_E_ARG, "unknown streaming %d", a->streaming);
+  if (a->collision < 0 || a->collision > LBM_COLL_LEARNED)
+    return fail(LBM_E_ARG, "unknown collision %d", a->collision);
+  const bool mask = a->cell_class != nullptr;
+  if (mask && (!a->classes || a->n_classes < 1 || a->n_classes > 256))
+    return fail(LBM_E_ARG, "cell_class given without a class table of 1..256 entries");
+  if (a->fallback_count_ptr_reserved != 0) return fail(LBM_E_ARG, "reserved field must be 0");
+  if (a->collision == LBM_COLL_BGK && !(a->tau > 0)) return fail(LBM_E_ARG, "BGK needs tau > 0");
+  if (a->collision == LBM_COLL_TRT && !(a->tau > 0 && a->tau_minus > 0))
+    return fail(LBM_E_ARG, "TRT needs tau > 0 and tau_minus > 0");
+  if (a->collision == LBM_COLL_KBC && !(a->tau > 0)) return fail(LBM_E_ARG, "KBC needs tau > 0");
+
+  switch (a->stencil) {
+    case LBM_D2Q9: fn = pick_kernel_d2q9(a->storage, a->collision, mask, a->streaming); break;
+    case LBM_D3Q19: fn = pick_kernel_d3q19(a->storage, a->collision, mask, a->streaming); break;
+    case LBM_D3Q27: fn = pick_kernel_d3q27(a->storage, a->collision, mask, a->streaming); break;
+    default: fn = nullptr;
+  }
+  if (!fn)
+    return fail(LBM_E_UNSUPPORTED, "no native kernel for stencil %d, storage %d, collision %d",
+                a->stencil, a->storage, a->collision);
+
+  p.fin = a->f_in;
+  p.fout = a->f_out;
+  p.cls = a->cell_class;
+  p.ct = a->classes;
+  p.bcf = a->bc_field;
+  p.n0 = (int)a->n[0];
+  p.n1 = (int)a->n[1];
+  p.n2 = (int)a->n[2];
+  p.s0 = (int)a->stride[1];
+  p.s1 = (int)a->stride[2];
+  p.sq = a->stride[0];
+  p.ncells = a->n[0] * a->n[1] * a->n[2];
+  const int bs = block_size_for(a->n[2]);
+  p.nb2 = (int)((a->n[2] + bs - 1) / bs);
+  const int64_t nb = (int64_t)p.nb2 * a->n[1] * a->n[0];
+  if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");
+  p.nblocks = (int)nb;
+  p.xcd_remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
+  p.tau_inv = a->collision == LBM_COLL_BGK ? 1.0 / a->tau : 0.0;
+  p.trt_p = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau) : 0.0;
+  p.trt_m = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau_minus) : 0.0;
+  p.beta = a->collision == LBM_COLL_KBC ? 1.0 / (2.0 * a->tau) : 0.0;
+  p.beta_inv = a->collision == LBM_COLL_KBC ? 2.0 * a->tau : 0.0;
+  p.rho0 = (a->storage == LBM_F16_SHIFTED || a->storage == LBM_BF16_SHIFTED) ? a->rho0_shift : 0.0;
+  grid = dim3((unsigned)nb);
+  block = dim3(bs);
+  return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+const char* lbm_last_error(void) { return g_err; }
+
+int64_t lbm_struct_size(int which) {
+  switch (which) {
+    case 0: return (int64_t)sizeof(lbm_step_args);
+    case 1: return (int64_t)sizeof(lbm_cell_class);
+    case 2: return (int64_t)sizeof(lbm_learned);
+  }
+  return -1;
+}
+
+int lbm_step(const lbm_step_args* a, void* stream) {
+  KParams p;
+  KernelFn fn;
+  dim3 grid, block;
+  int rc = prepare_step(a, p, fn, grid, block);
+  if (rc) return rc;
+  g_err[0] = 0;
+  hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
+  return check_launch("lbm_step");
+}
+
+int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
+  if (nsteps < 0) return fail(LBM_E_ARG, "negative step count");
+  if (nsteps == 0) return LBM_OK;
+  KParams p;
+  KernelFn fn;
+  dim3 grid, block;
+  int rc = prepare_step(a, p, fn, grid, block);
+  if (rc) return rc;
+  g_err[0] = 0;
+  const void* bufs[2] = {a->f_in, a->f_out};
+  for (int64_t k = 0; k < nsteps; ++k) {
+    p.fin = bufs[k & 1];
+    p.fout = (void*)bufs[(k + 1) & 1];
+    hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
+  }
+  return check_launch("lbm_run");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// utility kernels (grid-stride over cells; arrays are [Q][cells] views given
+// by the population stride and the spatial strides)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Grid {
+  int64_t n0, n1, n2, s0, s1, sq, cells;
+};
+
+__device__ __forceinline__ int64_t cell_offset(const Grid& g, int64_t c) {
+  const int64_t i2 = c % g.n2;
+  const int64_t r = c / g.n2;
+  const int64_t i1 = r % g.n1;
+  const int64_t i0 = r / g.n1;
+  return i0 * g.s0 + i1 * g.s1 + i2;
+}
+
+template <class S, class ST>
+__device__ __forceinline__ void load_cell(const Grid& g, const void* f, int64_t off, double rho0,
+                                          typename ST::C (&v)[S::Q]) {
+  using T = typename ST::T;
+  using C = typename ST::C;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
+    v[q] = ST::ld((const T*)f + q * g.sq, (uint32_t)off, sh);
+  }
+}
+
+template <class S, class ST, class IN>
+__global__ void k_equilibrium(Grid g, const IN* rho, const IN* u, double rho0, void* f) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    C r = (C)rho[c];
+    C uu[3] = {(C)u[c], (C)u[g.cells + c], S::D == 3 ? (C)u[2 * g.cells + c] : C(0)};
+    C fe[S::Q];
+    equilibrium<S, C>(r, uu, fe);
+    const int64_t off = cell_offset(g, c);
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) {
+      const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
+      ST::st((T*)f + q * g.sq, (uint32_t)off, fe[q], sh);
+    }
+  }
+}
+
+template <class S, class ST, class OUT>
+__global__ void k_decode(Grid g, const void* f, double rho0, OUT* out) {
+  using C = typename ST::C;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    C v[S::Q];
+    load_cell<S, ST>(g, f, cell_offset(g, c), rho0, v);
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) out[q * g.cells + c] = (OUT)v[q];
+  }
+}
+
+template <class S, class ST, class IN>
+__global__ void k_encode(Grid g, const IN* in, double rho0, void* f) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t off = cell_offset(g, c);
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) {
+      const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
+      ST::st((T*)f + q * g.sq, (uint32_t)off, (C)in[q * g.cells + c], sh);
+    }
+  }
+}
+
+// FP64 moments of one cell from its (decoded) populations
+template <class S, class C>
+__device__ __forceinline__ void moments_f64(const C (&v)[S::Q], double& rho, double (&j)[3]) {
+  rho = 0;
+  j[0] = j[1] = j[2] = 0;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const double x = (double)v[q];
+    rho += x;
+    j[0] += S::e[q][0] * x;
+    j[1] += S::e[q][1] * x;
+    j[2] += S::e[q][2] * x;
+  }
+}
+
+template <class S, class ST>
+__global__ void k_moments(Grid g, const void* f, double rho0, double* rho, double* u) {
+  using C = typename ST::C;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    C v[S::Q];
+    load_cell<S, ST>(g, f, cell_offset(g, c), rho0, v);
+    double r, j[3];
+    moments_f64<S, C>(v, r, j);
+    rho[c] = r;
+#pragma unroll
+    for (int d = 0; d < S::D; ++d) u[d * g.cells + c] = j[d] / r;
+  }
+}
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAX_BLOCKS = 2048;
+constexpr int RED_N = 5;
+
+// block-wide sum of RED_N doubles; result valid in thread 0
+__device__ void block_sum(double (&v)[RED_N]) {
+  __shared__ double sh[RED_N][RED_BLOCK / 64];
+#pragma unroll
+  for (int k = 0; k < RED_N; ++k) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < RED_N; ++k) {
+      double s = 0;
+      for (int w = 0; w < RED_BLOCK / 64; ++w) s += sh[k][w];
+      v[k] = s;
+    }
+  }
+}
+
+template <class S, class ST>
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce(Grid g, const void* f, double rho0,
+                                                      const uint8_t* mask, double* part) {
+  using C = typename ST::C;
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    if (mask && mask[c]) continue;
+    C v[S::Q];
+    load_cell<S, ST>(g, f, cell_offset(g, c), rho0, v);
+    double r, j[3];
+    moments_f64<S, C>(v, r, j);
+    acc[0] += r;
+    acc[1] += j[0];
+    acc[2] += j[1];
+    acc[3] += j[2];
+    const double ri = 1.0 / r;
+    acc[4] += 0.5 * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]) * ri * ri;
+  }
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < RED_N; ++k) part[blockIdx.x * RED_N + k] = acc[k];
+}
+
+// fixed-order final sum of per-block partials (deterministic)
+__global__ void k_finish(const double* part, int nparts, double* out, int nout) {
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nparts; b += RED_BLOCK)
+    for (int k = 0; k < nout; ++k) acc[k] += part[b * RED_N + k];
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nout; ++k) out[k] = acc[k];
+}
+
+// 6th-order periodic central difference along axis `a` (weights of
+// torch_gradient order 6, lettuce/util/utility.py:52-56)
+__device__ __forceinline__ double d6(const double* u, int64_t i0, int64_t i1, int64_t i2,
+                                     const int64_t (&n)[3], int a) {
+  const double w[3] = {3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+  double s = 0;
+  for (int k = 1; k <= 3; ++k) {
+    int64_t p[3] = {i0, i1, i2}, m[3] = {i0, i1, i2};
+    p[a] = (p[a] + k) % n[a];
+    m[a] = (m[a] - k + n[a]) % n[a];
+    const double up = u[(p[0] * n[1] + p[1]) * n[2] + p[2]];
+    const double um = u[(m[0] * n[1] + m[1]) * n[2] + m[2]];
+    s += w[k - 1] * (up - um);
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_enstrophy(int64_t n0, int64_t n1, int64_t n2, int dim,
+                                                         const double* u, double* part) {
+  const int64_t n[3] = {n0, n1, n2};
+  const int64_t cells = n0 * n1 * n2;
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  // lattice axes (x, y, z) -> storage axes: 2D (i1, i2), 3D (i0, i1, i2)
+  const int ax = dim == 2 ? 1 : 0, ay = dim == 2 ? 2 : 1, az = 2;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i2 = c % n2, r = c / n2, i1 = r % n1, i0 = r / n1;
+    const double* ux = u;
+    const double* uy = u + cells;
+    const double w3 = d6(ux, i0, i1, i2, n, ay) - d6(uy, i0, i1, i2, n, ax);
+    double s = w3 * w3;
+    if (dim == 3) {
+      const double* uz = u + 2 * cells;
+      const double w1 = d6(uz, i0, i1, i2, n, ay) - d6(uy, i0, i1, i2, n, az);
+      const double w2 = d6(ux, i0, i1, i2, n, az) - d6(uz, i0, i1, i2, n, ax);
+      s += w1 * w1 + w2 * w2;
+    }
+    acc[0] += s;
+  }
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < RED_N; ++k) part[blockIdx.x * RED_N + k] = acc[k];
+}
+
+template <class S, class ST>
+__global__ void __launch_bounds__(RED_BLOCK) k_link_force(Grid g, const void* f, double rho0,
+                                                          const int32_t* links, int64_t nl,
+                                                          double* part) {
+  using T = typename ST::T;
+  using C = typename ST::C;
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < nl;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = links[2 * l];
+    const int d = links[2 * l + 1];
+    const int64_t off = cell_offset(g, c);
+    double v = 0;
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q)
+      if (q == d) {
+        const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
+        v = (double)ST::ld((const T*)f + q * g.sq, (uint32_t)off, sh);
+        acc[0] += 2.0 * v * S::e[q][0];
+        acc[1] += 2.0 * v * S::e[q][1];
+        acc[2] += 2.0 * v * S::e[q][2];
+      }
+  }
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < RED_N; ++k) part[blockIdx.x * RED_N + k] = acc[k];
+}
+
+Grid make_grid(const int64_t n[3], const int64_t stride[4]) {
+  Grid g;
+  g.n0 = n[0];
+  g.n1 = n[1];
+  g.n2 = n[2];
+  g.sq = stride[0];
+  g.s0 = stride[1];
+  g.s1 = stride[2];
+  g.cells = n[0] * n[1] * n[2];
+  return g;
+}
+
+unsigned grid_blocks(int64_t cells, int block, int maxb) {
+  int64_t b = (cells + block - 1) / block;
+  if (b > maxb) b = maxb;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// dispatch helpers over (stencil, storage)
+template <template <class, class> class F, class... A>
+int with_lattice(int stencil, int storage, A... args) {
+#define LBM_CASE_ST(S)                                                 \
+  switch (storage) {                                                   \
+    case LBM_F32: return F<S, StF32>::run(args...);                    \
+    case LBM_F64: return F<S, StF64>::run(args...);                    \
+    case LBM_F16_SHIFTED: return F<S, StF16>::run(args...);            \
+    case LBM_BF16_SHIFTED: return F<S, StBF16>::run(args...);          \
+  }                                                                    \
+  break;
+  switch (stencil) {
+    case LBM_D2Q9: LBM_CASE_ST(D2Q9)
+    case LBM_D3Q19: LBM_CASE_ST(D3Q19)
+    case LBM_D3Q27: LBM_CASE_ST(D3Q27)
+  }
+#undef LBM_CASE_ST
+  return fail(LBM_E_ARG, "unknown stencil/storage %d/%d", stencil, storage);
+}
+
+template <class S, class ST>
+struct EqRun {
+  static int run(Grid g, const void* rho, const void* u, double rho0, void* f, hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, 256, 65536);
+    if (std::is_same<ST, StF64>::value)
+      hipLaunchKernelGGL((k_equilibrium<S, ST, double>), dim3(nb), dim3(256), 0, s, g,
+                         (const double*)rho, (const double*)u, rho0, f);
+    else
+      hipLaunchKernelGGL((k_equilibrium<S, ST, float>), dim3(nb), dim3(256), 0, s, g,
+                         (const float*)rho, (const float*)u, rho0, f);
+    return check_launch("lbm_equilibrium");
+  }
+};
+
+template <class S, class ST>
+struct DecodeRun {
+  static int run(Grid g, const void* f, double rho0, void* out, int f64, hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, 256, 65536);
+    if (f64)
+      hipLaunchKernelGGL((k_decode<S, ST, double>), dim3(nb), dim3(256), 0, s, g, f, rho0, (double*)out);
+    else
+      hipLaunchKernelGGL((k_decode<S, ST, float>), dim3(nb), dim3(256), 0, s, g, f, rho0, (float*)out);
+    return check_launch("lbm_decode");
+  }
+};
+
+template <class S, class ST>
+struct EncodeRun {
+  static int run(Grid g, const void* in, int f64, double rho0, void* f, hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, 256, 65536);
+    if (f64)
+      hipLaunchKernelGGL((k_encode<S, ST, double>), dim3(nb), dim3(256), 0, s, g, (const double*)in, rho0, f);
+    else
+      hipLaunchKernelGGL((k_encode<S, ST, float>), dim3(nb), dim3(256), 0, s, g, (const float*)in, rho0, f);
+    return check_launch("lbm_encode");
+  }
+};
+
+template <class S, class ST>
+struct MomRun {
+  static int run(Grid g, const void* f, double rho0, double* rho, double* u, hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, 256, 65536);
+    hipLaunchKernelGGL((k_moments<S, ST>), dim3(nb), dim3(256), 0, s, g, f, rho0, rho, u);
+    return check_launch("lbm_moments_f64");
+  }
+};
+
+template <class S, class ST>
+struct RedRun {
+  static int run(Grid g, const void* f, double rho0, const uint8_t* mask, double* out, double* ws,
+                 hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL((k_reduce<S, ST>), dim3(nb), dim3(RED_BLOCK), 0, s, g, f, rho0, mask, ws);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(RED_BLOCK), 0, s, (const double*)ws, (int)nb, out, RED_N);
+    return check_launch("lbm_reduce_f64");
+  }
+};
+
+template <class S, class ST>
+struct ForceRun {
+  static int run(Grid g, const void* f, double rho0, const int32_t* links, int64_t nl, double* out,
+                 double* ws, hipStream_t s) {
+    const unsigned nb = grid_blocks(nl, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL((k_link_force<S, ST>), dim3(nb), dim3(RED_BLOCK), 0, s, g, f, rho0, links, nl, ws);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(RED_BLOCK), 0, s, (const double*)ws, (int)nb, out, 3);
+    return check_launch("lbm_link_force_f64");
+  }
+};
+
+double shift_for(int storage, double rho0) {
+  return (storage == LBM_F16_SHIFTED || storage == LBM_BF16_SHIFTED) ? rho0 : 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_equilibrium(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                    const void* rho, const void* u, double rho0_shift, void* f, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!rho || !u || !f) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<EqRun>(stencil, storage, make_grid(n, stride), rho, u,
+                             shift_for(storage, rho0_shift), f, (hipStream_t)stream);
+}
+
+int lbm_decode(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+               const void* f, double rho0_shift, void* out, int32_t out_f64, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !out) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<DecodeRun>(stencil, storage, make_grid(n, stride), f,
+                                 shift_for(storage, rho0_shift), out, (int)out_f64, (hipStream_t)stream);
+}
+
+int lbm_encode(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+               const void* in, int32_t in_f64, double rho0_shift, void* f, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !in) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<EncodeRun>(stencil, storage, make_grid(n, stride), in, (int)in_f64,
+                                 shift_for(storage, rho0_shift), f, (hipStream_t)stream);
+}
+
+int lbm_moments_f64(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                    const void* f, double rho0_shift, double* rho, double* u, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !rho || !u) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<MomRun>(stencil, storage, make_grid(n, stride), f,
+                              shift_for(storage, rho0_shift), rho, u, (hipStream_t)stream);
+}
+
+int64_t lbm_reduce_workspace_bytes(const int64_t n[3]) {
+  (void)n;
+  return (int64_t)RED_MAX_BLOCKS * RED_N * (int64_t)sizeof(double);
+}
+
+int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                   const void* f, double rho0_shift, const uint8_t* mask, double* out, void* workspace,
+                   void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !out || !workspace) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<RedRun>(stencil, storage, make_grid(n, stride), f,
+                              shift_for(storage, rho0_shift), mask, out, (double*)workspace,
+                              (hipStream_t)stream);
+}
+
+int lbm_enstrophy_f64(int32_t stencil, const int64_t n[3], const double* u, double* out,
+                      void* workspace, void* stream) {
+  if (stencil_q(stencil) < 0) return fail(LBM_E_ARG, "unknown stencil %d", stencil);
+  if (!n || !u || !out || !workspace) return fail(LBM_E_ARG, "NULL pointer");
+  const int dim = stencil_d(stencil);
+  if (dim == 2 && n[0] != 1) return fail(LBM_E_ARG, "2D lattices need n[0] == 1");
+  if (n[0] < 1 || n[1] < 1 || n[2] < 1) return fail(LBM_E_ARG, "extents must be positive");
+  const int64_t cells = n[0] * n[1] * n[2];
+  const unsigned nb = grid_blocks(cells, RED_BLOCK, RED_MAX_BLOCKS);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_enstrophy, dim3(nb), dim3(RED_BLOCK), 0, s, n[0], n[1], n[2], dim, u,
+                     (double*)workspace);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(RED_BLOCK), 0, s, (const double*)workspace, (int)nb, out, 1);
+  return check_launch("lbm_enstrophy_f64");
+}
+
+int lbm_link_force_f64(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                       const void* f, double rho0_shift, const int32_t* links, int64_t n_links,
+                       double* out, void* workspace, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !out || !workspace || (n_links > 0 && !links)) return fail(LBM_E_ARG, "NULL pointer");
+  if (n_links < 0) return fail(LBM_E_ARG, "negative link count");
+  return with_lattice<ForceRun>(stencil, storage, make_grid(n, stride), f,
+                                shift_for(storage, rho0_shift), links, n_links, out, (double*)workspace,
+                                (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+// Device-side building blocks of the lattice-Boltzmann step for gfx950:
+// lattice tables, population storage codecs, equilibrium and collision
+// operators.  Everything here is computed per cell in registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lettuce_amd.h"
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Lattices.  Direction order, weights and opposites are lettuce's
+// (lettuce/ext/_stencil/d2q9.py:8-10, d3q19.py:8-13, d3q27.py:8-12) so that
+// population q here is population q of lettuce's flow.f.
+// e[q] holds the lattice components (x, y, z); go(q, a) is the grid offset of
+// direction q along storage axis a (i0, i1, i2).  2D lattices live on axes
+// (i1, i2) = (x, y) with i0 == 0.
+// ---------------------------------------------------------------------------
+struct D2Q9 {
+  static constexpr int Q = 9, D = 2, ID = LBM_D2Q9;
+  static constexpr int e[9][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},
+                                  {-1, 0, 0}, {0, -1, 0}, {1, 1, 0},
+                                  {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr double w[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+                                  1.0 / 9.0,  1.0 / 9.0,  1.0 / 36.0,
+                                  1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+};
+
+struct D3Q19 {
+  static constexpr int Q = 19, D = 3, ID = LBM_D3Q19;
+  static constexpr int e[19][3] = {
+      {0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+      {0, 0, 1},  {0, 0, -1},  {0, 1, 1},  {0, -1, -1}, {0, 1, -1},
+      {0, -1, 1}, {1, 0, 1},   {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+      {1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}};
+  static constexpr int opp[19] = {0,  2,  1,  4,  3,  6,  5,  8,  7, 10,
+                                  9, 12, 11, 14, 13, 16, 15, 18, 17};
+  static constexpr double w[19] = {
+      1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+      1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+};
+
+struct D3Q27 {
+  static constexpr int Q = 27, D = 3, ID = LBM_D3Q27;
+  static constexpr int e[27][3] = {
+      {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},
+      {0, 0, 1},   {0, 0, -1},  {0, 1, 1},   {0, -1, -1}, {0, 1, -1},
+      {0, -1, 1},  {1, 0, 1},   {-1, 0, -1}, {1, 0, -1},  {-1, 0, 1},
+      {1, 1, 0},   {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 1, 1},
+      {-1, -1, -1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1},
+      {1, -1, -1}, {-1, 1, 1}};
+  static constexpr int opp[27] = {0,  2,  1,  4,  3,  6,  5,  8,  7,
+                                  10, 9,  12, 11, 14, 13, 16, 15, 18,
+                                  17, 20, 19, 22, 21, 24, 23, 26, 25};
+  static constexpr double w[27] = {
+      8.0 / 27.0,   2.0 / 27.0,   2.0 / 27.0,   2.0 / 27.0,   2.0 / 27.0,
+      2.0 / 27.0,   2.0 / 27.0,   1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,
+      1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,
+      1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,   1.0 / 54.0,   1.0 / 216.0,
+      1.0 / 216.0,  1.0 / 216.0,  1.0 / 216.0,  1.0 / 216.0,  1.0 / 216.0,
+      1.0 / 216.0,  1.0 / 216.0};
+};
+
+// grid offset of direction q along storage axis a
+template <class S>
+__host__ __device__ constexpr int go(int q, int a) {
+  return S::D == 2 ? (a == 0 ? 0 : S::e[q][a - 1]) : S::e[q][a];
+}
+
+// ---------------------------------------------------------------------------
+// Storage codecs.  C is the arithmetic type.  Shifted formats store
+// f_q - w_q * rho0 so that the stored values are O(u) deviations.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+
+struct StF32 {
+  using T = float;
+  using C = float;
+  static constexpr bool shifted = false;
+  __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
+  __device__ static void st(T* p, uint32_t i, C v, C) {
+    __builtin_nontemporal_store(v, p + i);
+  }
+};
+struct StF64 {
+  using T = double;
+  using C = double;
+  static constexpr bool shifted = false;
+  __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
+  __device__ static void st(T* p, uint32_t i, C v, C) {
+    __builtin_nontemporal_store(v, p + i);
+  }
+};
+struct StF16 {
+  using T = _Float16;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const T* p, uint32_t i, C sh) {
+    return (float)p[i] + sh;
+  }
+  __device__ static void st(T* p, uint32_t i, C v, C sh) {
+    __builtin_nontemporal_store((_Float16)(v - sh), p + i);
+  }
+};
+struct StBF16 {
+  using T = uint16_t;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const T* p, uint32_t i, C sh) {
+    return bf16_to_f32(p[i]) + sh;
+  }
+  __device__ static void st(T* p, uint32_t i, C v, C sh) {
+    __builtin_nontemporal_store(f32_to_bf16_rne(v - sh), p + i);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Moments and the quadratic equilibrium
+// (lettuce/ext/_equilibrium/quadratic_equilibrium.py:11-24 with cs^2 = 1/3:
+//  feq = w rho ((2 e.u - u.u) / (2 cs^2) + (e.u / cs^2)^2 / 2 + 1)).
+// ---------------------------------------------------------------------------
+template <class S, class C>
+__device__ __forceinline__ void moments(const C (&f)[S::Q], C& rho, C (&u)[3]) {
+  C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    r += f[q];
+    if (S::e[q][0] != 0) jx += S::e[q][0] > 0 ? f[q] : -f[q];
+    if (S::e[q][1] != 0) jy += S::e[q][1] > 0 ? f[q] : -f[q];
+    if (S::D == 3 && S::e[q][2] != 0) jz += S::e[q][2] > 0 ? f[q] : -f[q];
+  }
+  rho = r;
+  const C ri = C(1) / r;
+  u[0] = jx * ri;
+  u[1] = jy * ri;
+  u[2] = S::D == 3 ? jz * ri : C(0);
+}
+
+template <class S, class C>
+__device__ __forceinline__ C eu_of(int q, const C (&u)[3]) {
+  C s = 0;
+  if (S::e[q][0] != 0) s += S::e[q][0] > 0 ? u[0] : -u[0];
+  if (S::e[q][1] != 0) s += S::e[q][1] > 0 ? u[1] : -u[1];
+  if (S::D == 3 && S::e[q][2] != 0) s += S::e[q][2] > 0 ? u[2] : -u[2];
+  return s;
+}
+
+template <class S, class C>
+__device__ __forceinline__ C feq_q(int q, C rho, const C (&u)[3], C uu) {
+  const C eu = eu_of<S, C>(q, u);
+  return C(S::w[q]) * rho * (C(1.5) * (C(2) * eu - uu) + C(4.5) * eu * eu + C(1));
+}
+
+template <class S, class C>
+__device__ __forceinline__ void equilibrium(C rho, const C (&u)[3], C (&feq)[S::Q]) {
+  const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) feq[q] = feq_q<S, C>(q, rho, u, uu);
+}
+
+// ---------------------------------------------------------------------------
+// Collision operators (the cell's own populations, in place)
+// ---------------------------------------------------------------------------
+template <class C>
+struct CollParams {
+  C tau_inv;      // BGK: 1 / tau
+  C trt_p;        // TRT: 1 / (2 tau_plus)
+  C trt_m;        // TRT: 1 / (2 tau_minus)
+  C beta;         // KBC: 1 / (2 tau)
+  C beta_inv;     // KBC: 2 tau
+};
+
+// BGK: f - (f - feq) / tau  (lettuce/ext/_collision/bgk_collision.py:17-22)
+template <class S, class C>
+__device__ __forceinline__ void collide_bgk(C (&f)[S::Q], const CollParams<C>& cp) {
+  C rho, u[3];
+  moments<S, C>(f, rho, u);
+  const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C fe = feq_q<S, C>(q, rho, u, uu);
+    f[q] = f[q] - cp.tau_inv * (f[q] - fe);
+  }
+}
+
+// TRT (lettuce/ext/_collision/trt_collision.py:16-27):
+// f - [(f+fb)-(feq+feqb)]/(2 tau+) - [(f-fb)-(feq-feqb)]/(2 tau-), fb = f[opp]
+template <class S, class C>
+__device__ __forceinline__ void collide_trt(C (&f)[S::Q], const CollParams<C>& cp) {
+  C rho, u[3], fe[S::Q];
+  moments<S, C>(f, rho, u);
+  equilibrium<S, C>(rho, u, fe);
+  C g[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const int o = S::opp[q];
+    const C dp = (f[q] + f[o]) - (fe[q] + fe[o]);
+    const C dm = (f[q] - f[o]) - (fe[q] - fe[o]);
+    g[q] = f[q] - dp * cp.trt_p - dm * cp.trt_m;
+  }
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) f[q] = g[q];
+}
+
+// KBC shear part Delta s from the raw second-order moments of f - feq.
+// lettuce normalises the moments by rho and multiplies back by m000
+// (kbc_collision.py:22-39, :41-94); in raw moments the s-part is:
+//  2D: s0 = -T, s1 = s3 = (T+N)/4, s2 = s4 = (T-N)/4, s5 = s7 = Pxy/4,
+//      s6 = s8 = -Pxy/4                                          (:76-94)
+//  3D: s0 = -T, s1,2 = (2Nxz - Nyz + T)/6, s3,4 = (2Nyz - Nxz + T)/6,
+//      s5,6 = (-Nxz - Nyz + T)/6, s7,8 = Pyz/4, s9,10 = -Pyz/4,
+//      s11,12 = Pxz/4, s13,14 = -Pxz/4, s15,16 = Pxy/4, s17,18 = -Pxy/4,
+//      corners s19..26 = 0                                       (:41-74)
+template <class S, class C>
+__device__ __forceinline__ void kbc_shear(const C (&df)[S::Q], C (&ds)[S::Q]) {
+  if constexpr (S::D == 2) {
+    C xx = 0, yy = 0, xy = 0;
+#pragma unroll
+    for (int q = 1; q < S::Q; ++q) {
+      const int ex = S::e[q][0], ey = S::e[q][1];
+      if (ex != 0) xx += df[q];
+      if (ey != 0) yy += df[q];
+      if (ex * ey != 0) xy += (ex * ey > 0) ? df[q] : -df[q];
+    }
+    const C T = xx + yy, N = xx - yy;
+    ds[0] = -T;
+    ds[1] = C(0.25) * (T + N);
+    ds[2] = C(0.25) * (T - N);
+    ds[3] = ds[1];
+    ds[4] = ds[2];
+    ds[5] = C(0.25) * xy;
+    ds[6] = -ds[5];
+    ds[7] = ds[5];
+    ds[8] = -ds[5];
+  } else {
+    C xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+#pragma unroll
+    for (int q = 1; q < S::Q; ++q) {
+      const int ex = S::e[q][0], ey = S::e[q][1], ez = S::e[q][2];
+      if (ex != 0) xx += df[q];
+      if (ey != 0) yy += df[q];
+      if (ez != 0) zz += df[q];
+      if (ex * ey != 0) xy += (ex * ey > 0) ? df[q] : -df[q];
+      if (ex * ez != 0) xz += (ex * ez > 0) ? df[q] : -df[q];
+      if (ey * ez != 0) yz += (ey * ez > 0) ? df[q] : -df[q];
+    }
+    const C T = xx + yy + zz, Nxz = xx - zz, Nyz = yy - zz;
+    const C sixth = C(1.0 / 6.0);
+    ds[0] = -T;
+    ds[1] = sixth * (C(2) * Nxz - Nyz + T);
+    ds[2] = ds[1];
+    ds[3] = sixth * (C(2) * Nyz - Nxz + T);
+    ds[4] = ds[3];
+    ds[5] = sixth * (-Nxz - Nyz + T);
+    ds[6] = ds[5];
+    ds[7] = C(0.25) * yz;
+    ds[8] = ds[7];
+    ds[9] = -ds[7];
+    ds[10] = ds[9];
+    ds[11] = C(0.25) * xz;
+    ds[12] = ds[11];
+    ds[13] = -ds[11];
+    ds[14] = ds[13];
+    ds[15] = C(0.25) * xy;
+    ds[16] = ds[15];
+    ds[17] = -ds[15];
+    ds[18] = ds[17];
+#pragma unroll
+    for (int q = 19; q < S::Q; ++q) ds[q] = C(0);
+  }
+}
+
+// KBC (lettuce/ext/_collision/kbc_collision.py:96-160):
+// dh = f - feq - ds, gamma = 1/beta - (2 - 1/beta) <ds dh/feq> / <dh dh/feq>
+// (gamma < 1e-15 or NaN -> 2), f' = f - beta (2 ds + gamma dh)
+template <class S, class C>
+__device__ __forceinline__ C kbc_gamma(const C (&ds)[S::Q], const C (&dh)[S::Q],
+                                       const C (&fe)[S::Q], const CollParams<C>& cp) {
+  C sum_s = 0, sum_h = 0;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C r = dh[q] / fe[q];
+    sum_s += ds[q] * r;
+    sum_h += dh[q] * r;
+  }
+  C gamma = cp.beta_inv - (C(2) - cp.beta_inv) * sum_s / sum_h;
+  if (!(gamma >= C(1e-15))) gamma = C(2);  // also catches NaN
+  return gamma;
+}
+
+template <class S, class C>
+__device__ __forceinline__ void collide_kbc(C (&f)[S::Q], const CollParams<C>& cp) {
+  C rho, u[3], fe[S::Q], df[S::Q], ds[S::Q], dh[S::Q];
+  moments<S, C>(f, rho, u);
+  equilibrium<S, C>(rho, u, fe);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) df[q] = f[q] - fe[q];
+  kbc_shear<S, C>(df, ds);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) dh[q] = df[q] - ds[q];
+  const C gamma = kbc_gamma<S, C>(ds, dh, fe, cp);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) f[q] = f[q] - cp.beta * (C(2) * ds[q] + gamma * dh[q]);
+}
+
+}  // namespace lbm

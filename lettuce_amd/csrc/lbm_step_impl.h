@@ -1,0 +1,290 @@
+// The fused stream + collide kernel.  One thread owns one cell; a block is a
+// run of cells along the contiguous axis i2, so every population load and
+// store of a wavefront is one coalesced 256-byte (FP32) transaction.
+//
+// Streaming follows lettuce's semantics exactly:
+//  - pre-streaming (pull), lettuce/_simulation.py:241-256:
+//      f_q(x) <- f_q(x - e_q) periodic, or f_q(x) if no_streaming_mask[q](x)
+//    plus the halfway bounce-back rule of
+//    examples/.../boundary/halfway_bounce_back_boundary.py:170-190:
+//      f_q(x) <- f*_opp(q)(x) when x is fluid and x - e_q is solid;
+//  - post-streaming (push) writes the same values from the source side, so
+//    each destination slot has exactly one writer (no races).
+// The per-cell operator (collision or boundary) is chosen by the cell class,
+// i.e. by lettuce's no_collision_mask (lettuce/_simulation.py:258-305).
+#pragma once
+
+#include "lbm_device.h"
+
+namespace lbm {
+
+struct KParams {
+  const void* fin;
+  void* fout;
+  const uint8_t* cls;
+  const lbm_cell_class* ct;
+  const double* bcf;
+  int n0, n1, n2;     // extents
+  int s0, s1;         // cell strides of i0 and i1 (elements); i2 stride is 1
+  long long sq;       // population stride (elements)
+  long long ncells;   // n0*n1*n2 (bc_field plane stride)
+  int nb2;            // blocks along i2
+  int nblocks;        // total blocks
+  int xcd_remap;      // 1: give each XCD a contiguous slab of blocks
+  double tau_inv, trt_p, trt_m, beta, beta_inv;
+  double rho0;
+};
+
+template <class C>
+__device__ __forceinline__ CollParams<C> coll_params(const KParams& p) {
+  CollParams<C> cp;
+  cp.tau_inv = (C)p.tau_inv;
+  cp.trt_p = (C)p.trt_p;
+  cp.trt_m = (C)p.trt_m;
+  cp.beta = (C)p.beta;
+  cp.beta_inv = (C)p.beta_inv;
+  return cp;
+}
+
+// periodic neighbour offsets along one axis: a[0] = i-1, a[1] = i, a[2] = i+1
+// (wrapped), each multiplied by the axis stride
+__device__ __forceinline__ void axis_offsets(int i, int n, int s, int (&a)[3]) {
+  a[0] = (i == 0 ? n - 1 : i - 1) * s;
+  a[1] = i * s;
+  a[2] = (i == n - 1 ? 0 : i + 1) * s;
+}
+
+// offset of x + sign * e_q for a cell with axis offset tables a0, a1, a2
+template <class S>
+__device__ __forceinline__ int nb_offset(int q, int sign, const int (&a0)[3],
+                                         const int (&a1)[3], const int (&a2)[3]) {
+  return a0[1 + sign * go<S>(q, 0)] + a1[1 + sign * go<S>(q, 1)] +
+         a2[1 + sign * go<S>(q, 2)];
+}
+
+template <class ST>
+__device__ __forceinline__ typename ST::C shift_of(double w, const KParams& p) {
+  using C = typename ST::C;
+  return ST::shifted ? (C)(w * p.rho0) : C(0);
+}
+
+__device__ __forceinline__ bool cls_solid(const KParams& p, int cell) {
+  return (p.ct[p.cls[cell]].flags & LBM_CLASS_SOLID) != 0;
+}
+
+// Post-streaming populations of the cell at offset `cell` whose neighbour
+// tables are a0/a1/a2, with the cell's own class fields (nsm, flags).
+template <class S, class ST, bool MASK>
+__device__ __forceinline__ void gather_pull(const KParams& p, int cell,
+                                            const int (&a0)[3], const int (&a1)[3],
+                                            const int (&a2)[3], uint32_t nsm,
+                                            uint32_t flags, typename ST::C (&f)[S::Q]) {
+  using T = typename ST::T;
+  const T* fin = (const T*)p.fin;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const typename ST::C sh = shift_of<ST>(S::w[q], p);
+    const int src = nb_offset<S>(q, -1, a0, a1, a2);
+    if (!MASK) {
+      f[q] = ST::ld(fin + q * p.sq, src, sh);
+    } else {
+      bool hw = false;
+      if ((flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID && q != 0)
+        hw = cls_solid(p, src);
+      if (hw) {
+        const int o = S::opp[q];
+        f[q] = ST::ld(fin + o * p.sq, cell, shift_of<ST>(S::w[o], p));
+      } else {
+        const int idx = ((nsm >> q) & 1u) ? cell : src;
+        f[q] = ST::ld(fin + q * p.sq, idx, sh);
+      }
+    }
+  }
+}
+
+template <class S, class ST>
+__device__ __forceinline__ void gather_own(const KParams& p, int cell,
+                                           typename ST::C (&f)[S::Q]) {
+  using T = typename ST::T;
+  const T* fin = (const T*)p.fin;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) f[q] = ST::ld(fin + q * p.sq, cell, shift_of<ST>(S::w[q], p));
+}
+
+template <class S, class C, int COLL>
+__device__ __forceinline__ void collide(C (&f)[S::Q], const CollParams<C>& cp) {
+  if constexpr (COLL == LBM_COLL_BGK) collide_bgk<S, C>(f, cp);
+  else if constexpr (COLL == LBM_COLL_TRT) collide_trt<S, C>(f, cp);
+  else if constexpr (COLL == LBM_COLL_KBC) collide_kbc<S, C>(f, cp);
+  // LBM_COLL_NONE: f unchanged
+}
+
+template <class S, class ST, int COLL, bool MASK, int MODE>
+__global__ void __launch_bounds__(256) step_kernel(const KParams p) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
+  constexpr bool PUSH = (MODE & LBM_POST_STREAMING) != 0;
+
+  // XCD-aware block order: hardware places block b on XCD b % 8, so block b
+  // takes logical tile (b % 8) * (nblocks / 8) + b / 8 and each XCD sweeps a
+  // contiguous slab whose halo rows stay in its own L2.
+  int b = blockIdx.x;
+  if (p.xcd_remap) b = (b & 7) * (p.nblocks >> 3) + (b >> 3);
+  const int b2 = b % p.nb2;
+  const int r = b / p.nb2;
+  const int i1 = r % p.n1;
+  const int i0 = r / p.n1;
+  const int i2 = b2 * blockDim.x + threadIdx.x;
+  if (i2 >= p.n2) return;
+
+  int a0[3], a1[3], a2[3];
+  axis_offsets(i0, p.n0, p.s0, a0);
+  axis_offsets(i1, p.n1, p.s1, a1);
+  axis_offsets(i2, p.n2, 1, a2);
+  const int cell = a0[1] + a1[1] + a2[1];
+
+  uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
+  if (MASK) {
+    const lbm_cell_class& K = p.ct[p.cls[cell]];
+    nsm = K.no_stream;
+    flags = K.flags;
+    op = K.op;
+  }
+
+  C f[S::Q];
+  if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
+  else gather_own<S, ST>(p, cell, f);
+
+  const CollParams<C> cp = coll_params<C>(p);
+  if (!MASK || op == LBM_OP_COLLIDE) {
+    collide<S, C, COLL>(f, cp);
+  } else if (op == LBM_OP_BOUNCE_BACK) {
+    // fullway bounce-back: f_q <- f_opp(q) (bounce_back_boundary.py:17-18)
+    C g[S::Q];
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) g[q] = f[S::opp[q]];
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) f[q] = g[q];
+  } else if (op == LBM_OP_EQUILIBRIUM) {
+    // f <- feq(rho_bc, u_bc) (equilibrium_boundary_pu.py:79-84)
+    const lbm_cell_class& K = p.ct[p.cls[cell]];
+    C rho, u[3];
+    if (flags & LBM_CLASS_BC_FIELD) {
+      rho = (C)p.bcf[cell];
+      u[0] = (C)p.bcf[p.ncells + cell];
+      u[1] = (C)p.bcf[2 * p.ncells + cell];
+      u[2] = (C)p.bcf[3 * p.ncells + cell];
+    } else {
+      rho = (C)K.rho;
+      u[0] = (C)K.u[0];
+      u[1] = (C)K.u[1];
+      u[2] = (C)K.u[2];
+    }
+    equilibrium<S, C>(rho, u, f);
+  } else if (op == LBM_OP_OUTLET) {
+    // f <- feq(rho_out, u(x - dir)) (equilibrium_outlet_p.py:63-73); the
+    // neighbour's velocity is taken from its streamed populations.
+    const lbm_cell_class& K = p.ct[p.cls[cell]];
+    const int j0 = i0 - K.dir[0], j1 = i1 - K.dir[1], j2 = i2 - K.dir[2];
+    const int w0 = j0 < 0 ? j0 + p.n0 : (j0 >= p.n0 ? j0 - p.n0 : j0);
+    const int w1 = j1 < 0 ? j1 + p.n1 : (j1 >= p.n1 ? j1 - p.n1 : j1);
+    const int w2 = j2 < 0 ? j2 + p.n2 : (j2 >= p.n2 ? j2 - p.n2 : j2);
+    int c0[3], c1[3], c2[3];
+    axis_offsets(w0, p.n0, p.s0, c0);
+    axis_offsets(w1, p.n1, p.s1, c1);
+    axis_offsets(w2, p.n2, 1, c2);
+    const int nb = c0[1] + c1[1] + c2[1];
+    C g[S::Q];
+    if (PULL) {
+      const lbm_cell_class& KN = p.ct[p.cls[nb]];
+      gather_pull<S, ST, true>(p, nb, c0, c1, c2, KN.no_stream, KN.flags, g);
+    } else {
+      gather_own<S, ST>(p, nb, g);
+    }
+    C rho_n, u[3];
+    moments<S, C>(g, rho_n, u);
+    equilibrium<S, C>((C)K.rho, u, f);
+  }
+  // LBM_OP_KEEP: unchanged
+
+  T* fout = (T*)p.fout;
+  if (!PUSH) {
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) ST::st(fout + q * p.sq, cell, f[q], shift_of<ST>(S::w[q], p));
+  } else if (!MASK) {
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q)
+      ST::st(fout + q * p.sq, nb_offset<S>(q, +1, a0, a1, a2), f[q], shift_of<ST>(S::w[q], p));
+  } else {
+    // Push with the pull rule evaluated at the destination d = x + e_q:
+    //   d fluid and x solid (halfway bounce-back)  -> d fills itself
+    //   no_streaming_mask[q](d)                     -> d keeps its own value
+    //   otherwise                                   -> d receives f_q(x)
+    const bool self_solid = (flags & LBM_CLASS_SOLID) != 0;
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) {
+      const C sh = shift_of<ST>(S::w[q], p);
+      // (a) to the neighbour
+      const int dst = nb_offset<S>(q, +1, a0, a1, a2);
+      const lbm_cell_class& KD = p.ct[p.cls[dst]];
+      const bool d_solid = (KD.flags & LBM_CLASS_SOLID) != 0;
+      const bool d_keeps = ((KD.no_stream >> q) & 1u) != 0;
+      const bool d_bounce = q != 0 && self_solid && !d_solid;
+      if (!d_keeps && !d_bounce) ST::st(fout + q * p.sq, dst, f[q], sh);
+      // (b) the cell's own slot q
+      bool own_bounce = false;
+      if (q != 0 && !self_solid && (flags & LBM_CLASS_NEAR_SOLID))
+        own_bounce = cls_solid(p, nb_offset<S>(q, -1, a0, a1, a2));
+      if (own_bounce) ST::st(fout + q * p.sq, cell, f[S::opp[q]], sh);
+      else if ((nsm >> q) & 1u) ST::st(fout + q * p.sq, cell, f[q], sh);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+using KernelFn = void (*)(const KParams);
+
+template <class S, class ST, int COLL, bool MASK>
+KernelFn pick_mode(int mode) {
+  switch (mode) {
+    case 0: return step_kernel<S, ST, COLL, MASK, 0>;
+    case 1: return step_kernel<S, ST, COLL, MASK, 1>;
+    case 2: return step_kernel<S, ST, COLL, MASK, 2>;
+    case 3: return step_kernel<S, ST, COLL, MASK, 3>;
+  }
+  return nullptr;
+}
+
+template <class S, class ST, int COLL>
+KernelFn pick_mask(bool mask, int mode) {
+  return mask ? pick_mode<S, ST, COLL, true>(mode) : pick_mode<S, ST, COLL, false>(mode);
+}
+
+template <class S, class ST>
+KernelFn pick_coll(int coll, bool mask, int mode) {
+  switch (coll) {
+    case LBM_COLL_NONE: return pick_mask<S, ST, LBM_COLL_NONE>(mask, mode);
+    case LBM_COLL_BGK: return pick_mask<S, ST, LBM_COLL_BGK>(mask, mode);
+    case LBM_COLL_TRT: return pick_mask<S, ST, LBM_COLL_TRT>(mask, mode);
+    case LBM_COLL_KBC:
+      if constexpr (S::ID == LBM_D3Q19) return nullptr;  // lettuce: D2Q9/D3Q27 only
+      else return pick_mask<S, ST, LBM_COLL_KBC>(mask, mode);
+  }
+  return nullptr;
+}
+
+template <class S>
+KernelFn pick_kernel(int storage, int coll, bool mask, int mode) {
+  switch (storage) {
+    case LBM_F32: return pick_coll<S, StF32>(coll, mask, mode);
+    case LBM_F64: return pick_coll<S, StF64>(coll, mask, mode);
+    case LBM_F16_SHIFTED: return pick_coll<S, StF16>(coll, mask, mode);
+    case LBM_BF16_SHIFTED: return pick_coll<S, StBF16>(coll, mask, mode);
+  }
+  return nullptr;
+}
+
+}  // namespace lbm

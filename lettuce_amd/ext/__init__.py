@@ -1,0 +1,4 @@
+"""Operators, flows and reporters (lettuce.ext counterparts)."""
+from ._operators import *  # noqa: F401,F403
+from ._flows import *  # noqa: F401,F403
+from ._reporter import *  # noqa: F401,F403

@@ -1,0 +1,347 @@
+"""Flows (initial conditions and boundary sets).
+
+Same constructors and initial fields as lettuce's ExtFlow,
+TaylorGreenVortex, DecayingTurbulence and Obstacle
+(lettuce/ext/_flows/_ext_flow.py:8-42, taylorgreen.py:16-98,
+decayingturbulence.py:23-189, obstacle.py:16-125).  Initial fields are set
+up with torch/numpy on the host or device (setup, not the step); the
+equilibrium initialisation runs through the native kernel.
+
+``Obstacle`` additionally accepts ``outlet`` ("antibounceback" as in
+lettuce, or "equilibrium" for EquilibriumOutletP), ``solid``
+("fullway" as in lettuce, or "halfway" for halfway bounce-back) and
+``walls`` (halfway solid rows at the lateral domain edges).
+"""
+import warnings
+from abc import ABC, abstractmethod
+from typing import List, Optional, Union
+
+import numpy as np
+import torch
+
+from .._flow import Flow
+from .._stencil import D2Q9, D3Q19, Stencil
+from .._unit import UnitConversion
+from ._operators import (AntiBounceBackOutlet, BounceBackBoundary,
+                         EquilibriumBoundaryPU, EquilibriumOutletP,
+                         HalfwayBounceBackBoundary, QuadraticEquilibrium)
+
+__all__ = ["ExtFlow", "TaylorGreenVortex", "DecayingTurbulence", "Obstacle"]
+
+
+class ExtFlow(Flow, ABC):
+    def __init__(self, context, resolution: Union[int, List[int]],
+                 reynolds_number, mach_number,
+                 stencil: Optional[Stencil] = None, equilibrium=None):
+        resolution = self.make_resolution(resolution, stencil)
+        if len(resolution) not in (2, 3):
+            raise ValueError("lettuce_amd supports 2D and 3D flows")
+        stencil = stencil or (D2Q9() if len(resolution) == 2 else D3Q19())
+        stencil = stencil() if callable(stencil) else stencil
+        equilibrium = equilibrium or QuadraticEquilibrium()
+        Flow.__init__(self, context, resolution,
+                      self.make_units(reynolds_number, mach_number, resolution),
+                      stencil, equilibrium)
+
+    @abstractmethod
+    def make_resolution(self, resolution, stencil=None) -> List[int]:
+        ...
+
+    @abstractmethod
+    def make_units(self, reynolds_number, mach_number, resolution
+                   ) -> UnitConversion:
+        ...
+
+
+def _periodic_grid(resolution, context):
+    """linspace(0, 2 pi) without the endpoint, in the context dtype."""
+    axes = tuple(torch.linspace(0, 2 * torch.pi * (1 - 1 / n), steps=n,
+                                device=context.device, dtype=context.dtype)
+                 for n in resolution)
+    return torch.meshgrid(*axes, indexing="ij")
+
+
+class TaylorGreenVortex(ExtFlow):
+    def __init__(self, context, resolution, reynolds_number, mach_number,
+                 stencil: Optional[Stencil] = None, equilibrium=None,
+                 initialize_fneq: bool = True):
+        self.initialize_fneq = initialize_fneq
+        if stencil is None and not isinstance(resolution, list):
+            warnings.warn("Requiring information about dimensionality! "
+                          "Setting dimension to 2.", UserWarning)
+            self.stencil = D2Q9()
+        else:
+            self.stencil = stencil() if callable(stencil) else stencil
+        ExtFlow.__init__(self, context, resolution, reynolds_number,
+                         mach_number, stencil, equilibrium)
+
+    def make_resolution(self, resolution, stencil=None) -> List[int]:
+        if isinstance(resolution, int):
+            return [resolution] * self.stencil.d
+        return list(resolution)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return UnitConversion(
+            reynolds_number=reynolds_number, mach_number=mach_number,
+            characteristic_length_lu=resolution[0] / (2 * torch.pi),
+            characteristic_length_pu=1, characteristic_velocity_pu=1)
+
+    @property
+    def grid(self):
+        return _periodic_grid(self.resolution, self.context)
+
+    def initial_pu(self):
+        return self.analytic_solution(t=0)
+
+    def analytic_solution(self, t: float):
+        """2D: u = (cos x sin y, -sin x cos y) e^{-2 nu t},
+        p = -(cos 2x + cos 2y)/4 e^{-4 nu t}; 3D initial field
+        (taylorgreen.py:66-94)."""
+        x = self.grid
+        nu = self.context.convert_to_tensor(self.units.viscosity_pu)
+        if len(self.resolution) == 2:
+            decay = torch.exp(-2 * nu * t)
+            u = torch.stack([torch.cos(x[0]) * torch.sin(x[1]) * decay,
+                             -torch.sin(x[0]) * torch.cos(x[1]) * decay])
+            p = -torch.stack([0.25 * (torch.cos(2 * x[0]) + torch.cos(2 * x[1]))
+                              * torch.exp(-4 * nu * t)])
+        else:
+            u = torch.stack([torch.sin(x[0]) * torch.cos(x[1]) * torch.cos(x[2]),
+                             -torch.cos(x[0]) * torch.sin(x[1]) * torch.cos(x[2]),
+                             torch.zeros_like(x[0])])
+            p = torch.stack([1 / 16. * (torch.cos(2 * x[0]) + torch.cos(2 * x[1]))
+                             * (torch.cos(2 * x[2]) + 2)])
+        return p, u
+
+    @property
+    def post_boundaries(self):
+        return []
+
+
+class DecayingTurbulence(ExtFlow):
+    """Decaying turbulence with E(k) ~ k^4 exp(-2 (k/k0)^2), random phases
+    from numpy's legacy RNG, a solenoidal projection with the modified
+    wavenumber and velocity scaled to the requested energy
+    (decayingturbulence.py:54-170)."""
+
+    def __init__(self, context, resolution, reynolds_number, mach_number,
+                 k0=20, ic_energy=0.5, stencil=None, equilibrium=None,
+                 initialize_pressure: bool = True,
+                 initialize_fneq: bool = True,
+                 randseed: Optional[int] = None):
+        self.initialize_pressure = initialize_pressure
+        self.initialize_fneq = initialize_fneq
+        self.randseed = randseed
+        self.k0 = k0
+        self.ic_energy = ic_energy
+        self.wavenumbers = []
+        self.spectrum = []
+        stencil = stencil or (D2Q9() if len(resolution) == 2 else D3Q19())
+        stencil = stencil() if callable(stencil) else stencil
+        if stencil.d != 2:
+            self.initialize_pressure = False
+        super().__init__(context, resolution, reynolds_number, mach_number,
+                         stencil, equilibrium)
+
+    def make_resolution(self, resolution, stencil=None) -> List[int]:
+        if isinstance(resolution, int):
+            return [resolution] * stencil.d
+        return list(resolution)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return UnitConversion(
+            reynolds_number=reynolds_number, mach_number=mach_number,
+            characteristic_length_lu=resolution[0],
+            characteristic_length_pu=2 * np.pi,
+            characteristic_velocity_pu=None)
+
+    def _wavenumbers(self):
+        dims = tuple(self.resolution)
+        freqs = [np.fft.fftfreq(n, d=1 / n) for n in dims]
+        wavenumber = np.meshgrid(*freqs)
+        norms = np.linalg.norm(wavenumber, axis=0)
+        self.wavenumbers = np.arange(int(np.max(norms)))
+        shells = ((norms[..., None] > self.wavenumbers - 0.5)
+                  & (norms[..., None] <= self.wavenumbers + 0.5))
+        return norms, wavenumber, shells
+
+    def _spectrum(self):
+        norms, wavenumber, shells = self._wavenumbers()
+        ek = norms ** 4 * np.exp(-2 * (norms / self.k0) ** 2)
+        ek /= np.sum(ek)
+        ek *= self.ic_energy
+        self.spectrum = np.sum(ek[..., None] * shells,
+                               axis=tuple(range(self.stencil.d)))
+        return ek, wavenumber
+
+    def _velocity(self, ek, wavenumber):
+        d = self.stencil.d
+        dims = tuple(self.resolution)
+        axes = tuple(range(d))
+        dx = self.units.convert_length_to_pu(1.0)
+        np.random.seed(self.randseed)
+        phases = np.random.random(np.array(wavenumber).shape) * 2 * np.pi + 0j
+        uh = [np.fft.fftn(phases[k], axes=axes) for k in range(d)]
+        re = [c.real for c in uh]
+        im = [c.imag for c in uh]
+        for k in range(d):
+            re[k].ravel()[0] = 0
+            im[k].ravel()[0] = 0
+        amp = [np.sqrt(2 / d * ek / (im[k] ** 2 + re[k] ** 2 + 1.e-15))
+               for k in range(d)]
+        re = [amp[k] * re[k] for k in range(d)]
+        im = [amp[k] * im[k] for k in range(d)]
+        for k in range(d):
+            re[k].ravel()[0] = 0
+            im[k].ravel()[0] = 0
+        # remove divergence with the modified wavenumber sin(k dx)/dx of the
+        # second-order central difference
+        kmod = [np.sin(wavenumber[k] * dx) / dx for k in range(d)]
+        knorm = np.linalg.norm(kmod, axis=0) + 1e-16
+        div_re = np.zeros(dims)
+        div_im = np.zeros(dims)
+        for k in range(d):
+            div_re += kmod[k] * re[k]
+            div_im += kmod[k] * im[k]
+        re = [re[k] - div_re * kmod[k] / knorm ** 2 for k in range(d)]
+        im = [im[k] - div_im * kmod[k] / knorm ** 2 for k in range(d)]
+        for k in range(d):
+            re[k].ravel()[0] = 0
+            im[k].ravel()[0] = 0
+        e_kin = 0.5 * np.sum([np.sum(re[k] ** 2 + im[k] ** 2) for k in range(d)])
+        factor = np.sqrt(self.ic_energy / e_kin)
+        re = [r * factor for r in re]
+        im = [i * factor for i in im]
+        if d == 3:
+            norm = (self.resolution[0] * dx ** (1 - d)
+                    * np.sqrt(self.units.characteristic_length_pu))
+        else:
+            norm = self.resolution[0] / dx
+        return np.asarray([(np.fft.ifftn(re[k] + im[k] * 1.0j, axes=axes)
+                            * norm).real for k in range(d)])
+
+    def initial_pu(self):
+        ek, wavenumber = self._spectrum()
+        u = self._velocity(ek, wavenumber)
+        self.units.characteristic_velocity_pu = np.linalg.norm(u, axis=0).max()
+        p = np.zeros(tuple(self.resolution))[None, ...]
+        return p, u
+
+    @property
+    def energy_spectrum(self):
+        return self.spectrum, self.wavenumbers
+
+    @property
+    def grid(self):
+        return _periodic_grid(self.resolution, self.context)
+
+    @property
+    def post_boundaries(self):
+        return []
+
+
+class Obstacle(ExtFlow):
+    """Flow past an obstacle mask: equilibrium inlet at x = 0, an outlet at
+    x = -1 and a solid mask (lettuce/ext/_flows/obstacle.py:16-125)."""
+
+    def __init__(self, context, resolution, reynolds_number, mach_number,
+                 domain_length_x, char_length=1, char_velocity=1,
+                 stencil: Optional[Stencil] = None, equilibrium=None,
+                 outlet: str = "antibounceback", solid: str = "fullway",
+                 walls: bool = False):
+        self.char_length_lu = resolution[0] / domain_length_x * char_length
+        self.char_length = char_length
+        self.char_velocity = char_velocity
+        if outlet not in ("antibounceback", "equilibrium"):
+            raise ValueError(f"unknown outlet {outlet!r}")
+        if solid not in ("fullway", "halfway"):
+            raise ValueError(f"unknown solid boundary {solid!r}")
+        self.outlet = outlet
+        self.solid = solid
+        self.walls = walls
+        self.resolution = self.make_resolution(resolution, stencil)
+        self._mask = torch.zeros(self.resolution, dtype=torch.bool)
+        ExtFlow.__init__(self, context, resolution, reynolds_number,
+                         mach_number, stencil, equilibrium)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return UnitConversion(
+            reynolds_number=reynolds_number, mach_number=mach_number,
+            characteristic_length_lu=self.char_length_lu,
+            characteristic_length_pu=self.char_length,
+            characteristic_velocity_pu=self.char_velocity)
+
+    def make_resolution(self, resolution, stencil=None) -> List[int]:
+        if isinstance(resolution, int):
+            return [resolution] * (stencil.d if stencil else 2)
+        return list(resolution)
+
+    @property
+    def mask(self):
+        return self._mask
+
+    @mask.setter
+    def mask(self, m):
+        if not (isinstance(m, (np.ndarray, torch.Tensor))
+                and all(m.shape[k] == self.resolution[k]
+                        for k in range(len(self.resolution)))):
+            raise ValueError("mask must match the resolution")
+        self._mask = self.context.convert_to_tensor(m, dtype=torch.bool)
+
+    def wall_mask(self) -> torch.Tensor:
+        """Solid rows at both ends of every lateral axis, excluding the inlet
+        and outlet columns."""
+        m = torch.zeros(self.resolution, dtype=torch.bool,
+                        device=self._mask.device)
+        if self.walls:
+            for a in range(1, len(self.resolution)):
+                lo = [slice(1, -1)] + [slice(None)] * (len(self.resolution) - 1)
+                hi = list(lo)
+                lo[a] = 0
+                hi[a] = -1
+                m[tuple(lo)] = True
+                m[tuple(hi)] = True
+        return m
+
+    def solid_mask(self) -> torch.Tensor:
+        return self._mask.to(torch.bool) | self.wall_mask().to(self._mask.device)
+
+    def initial_pu(self):
+        p = np.zeros(tuple(self.resolution), dtype=float)[None, ...]
+        u_char = self.units.characteristic_velocity_pu * self._unit_vector()
+        u_char = u_char.reshape([-1] + [1] * len(self.resolution))
+        solid = self.solid_mask().cpu()
+        u = (~solid).to(u_char.dtype) * u_char
+        return p, u
+
+    @property
+    def grid(self):
+        xyz = tuple(self.units.convert_length_to_pu(torch.arange(n))
+                    for n in self.resolution)
+        return torch.meshgrid(*xyz, indexing="ij")
+
+    @property
+    def post_boundaries(self):
+        x = self.grid[0]
+        inlet = EquilibriumBoundaryPU(
+            flow=self, context=self.context, mask=torch.abs(x) < 1e-6,
+            velocity=self.units.characteristic_velocity_pu * self._unit_vector())
+        if self.outlet == "antibounceback":
+            outlet = AntiBounceBackOutlet(self._unit_vector().tolist(), self)
+        else:
+            outlet = EquilibriumOutletP([int(c) for c in self._unit_vector().tolist()],
+                                        self)
+        boundaries = [inlet, outlet]
+        if self.solid == "fullway":
+            boundaries.append(BounceBackBoundary(self.solid_mask()))
+        return boundaries
+
+    @property
+    def post_streaming_boundaries(self):
+        if self.solid == "halfway":
+            return [HalfwayBounceBackBoundary(self.context, self,
+                                              self.solid_mask())]
+        return []
+
+    def _unit_vector(self, i=0):
+        return torch.eye(len(self.resolution))[i]

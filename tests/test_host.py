@@ -1,0 +1,154 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+everything include/lettuce_amd.h declares, the ctypes mirrors match the C
+structs, and the boundary masks / cell classes / halfway links are
+bit-exact against lettuce's masks (golden) and the oracle."""
+import ctypes
+import os
+import re
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+import lettuce_amd as lt
+from lettuce_amd import _native as N
+from lettuce_amd._classes import (OpSpec, build_cell_classes, build_masks,
+                                  expand_classes, halfway_links)
+from oracle import lettuce_torch as O
+from tests.conftest import ROOT, golden
+
+HEADER = os.path.join(ROOT, "include", "lettuce_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(lbm_\w+)\s*\(",
+                                 text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 12
+    lib = N.lib()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.lbm_abi_version() == N.ABI_VERSION
+
+
+def test_struct_mirrors_match_library():
+    lib = N.lib()
+    assert lib.lbm_struct_size(0) == ctypes.sizeof(N.LbmStepArgs)
+    assert lib.lbm_struct_size(1) == ctypes.sizeof(N.LbmCellClass) == 48
+    assert lib.lbm_struct_size(2) == ctypes.sizeof(N.LbmLearned)
+
+
+def test_step_rejects_bad_arguments_without_a_gpu():
+    """Argument validation happens before any HIP call."""
+    lib = N.lib()
+    a = N.LbmStepArgs()
+    a.abi_version = 99
+    assert lib.lbm_step(ctypes.byref(a), None) == -2
+    a.abi_version = N.ABI_VERSION
+    a.stencil = 7
+    assert lib.lbm_step(ctypes.byref(a), None) == -1
+    assert b"stencil" in lib.lbm_last_error()
+    a.stencil = N.D3Q19
+    a.n[:] = (4, 4, 4)
+    a.stride[:] = (64, 16, 4, 1)
+    a.f_in = a.f_out = 4096
+    assert lib.lbm_step(ctypes.byref(a), None) == -1
+    assert b"distinct" in lib.lbm_last_error()
+    a.f_out = 8192
+    a.collision = N.COLL_KBC
+    a.tau = 0.6
+    assert lib.lbm_step(ctypes.byref(a), None) == -3   # KBC has no D3Q19 kernel
+    a.collision = N.COLL_BGK
+    a.stride[:] = (10, 16, 4, 1)                        # populations overlap
+    assert lib.lbm_step(ctypes.byref(a), None) == -1
+
+
+def _cpu_flow(stencil, res):
+    return SimpleNamespace(stencil=stencil, resolution=list(res),
+                           context=lt.Context("cpu"), torch_stencil=None)
+
+
+def test_masks_and_classes_match_lettuce_obstacle():
+    """ncm / nsm built by lettuce_amd equal lettuce's (golden) bit for bit,
+    and the cell-class map reproduces both."""
+    g = golden("g6_obstacle_eqoutlet.npz")
+    ctx = lt.Context("cpu")
+    flow = _cpu_flow(lt.D2Q9(), (48, 24))
+    x = torch.arange(48, dtype=torch.float64) / 12.0
+    inlet = (torch.abs(x) < 1e-6)[:, None].expand(48, 24)
+    bounds = [lt.EquilibriumBoundaryPU(ctx, flow, inlet, [1.0, 0.0]),
+              lt.EquilibriumOutletP([1, 0], flow),
+              lt.BounceBackBoundary(torch.from_numpy(g["mask"]))]
+    ncm, nsm = build_masks(9, [48, 24], [], bounds, [], ctx)
+    assert np.array_equal(ncm.numpy(), g["ncm"])
+    assert np.array_equal(nsm.numpy(), g["nsm"])
+    ops = {0: OpSpec(op=N.OP_COLLIDE), 1: OpSpec(op=N.OP_EQUILIBRIUM),
+           2: OpSpec(op=N.OP_OUTLET), 3: OpSpec(op=N.OP_BOUNCE_BACK)}
+    cc = build_cell_classes(ncm, nsm, ops, lt.D2Q9().e)
+    r_ncm, r_nsm, r_solid = expand_classes(cc, 9)
+    assert torch.equal(r_ncm, ncm)
+    assert torch.equal(r_nsm, (nsm == 1).to(torch.uint8))
+    assert not r_solid.any()
+    assert cc.n_classes <= 256
+
+
+def test_nsm_predicate_is_equality_with_one():
+    """lettuce's torch step keeps population q only where nsm[q] == 1
+    (lettuce/_simulation.py:254); with two pre-boundaries nsm starts at 2."""
+    ctx = lt.Context("cpu")
+    m = torch.zeros(6, 6, dtype=torch.bool)
+    m[1:3, 1:3] = True
+
+    class B:
+        def __init__(self, mask):
+            self.mask = mask
+
+        def make_no_collision_mask(self, shape, context):
+            return self.mask
+
+        def make_no_streaming_mask(self, shape, context):
+            return self.mask
+
+    ncm, nsm = build_masks(9, [6, 6], [B(m), B(~m)], [], [], ctx)
+    assert int(nsm.min()) == 3 and int(nsm.max()) == 3   # 2 | 1
+    cc = build_cell_classes(ncm, nsm, {}, lt.D2Q9().e)
+    _, r_nsm, _ = expand_classes(cc, 9)
+    assert not r_nsm.any()
+
+
+def test_halfway_links_match_oracle_link_list():
+    s = lt.D2Q9()
+    g = torch.Generator().manual_seed(3)
+    solid = torch.rand(12, 9, generator=g) < 0.2
+    links = halfway_links(solid, s.e, s.opposite)
+    oflow = O.Flow(O.Context("cpu", torch.float64), [12, 9],
+                   O.UnitConversion(10.0), O.D2Q9())
+    hw = O.HalfwayBounceBackBoundary(oflow.context, oflow, solid)
+    ref = {(int(r[1]) * 9 + int(r[2]), int(r[0])) for r in hw.f_index}
+    ours = {(int(c), int(d)) for c, d in links.tolist()}
+    assert ours == ref
+    assert links.shape[0] == len(ref)
+    # ordered by (cell, direction)
+    key = links[:, 0].long() * 64 + links[:, 1].long()
+    assert torch.all(key[1:] > key[:-1])
+
+
+def test_halfway_classes_flags():
+    s = lt.D3Q27()
+    solid = torch.zeros(5, 5, 5, dtype=torch.bool)
+    solid[2, 2, 2] = True
+    ncm = torch.full((5, 5, 5), 0, dtype=torch.uint8)
+    ncm[solid] = 1
+    nsm = torch.zeros((27, 5, 5, 5), dtype=torch.uint8)
+    nsm[:, solid] = 1
+    cc = build_cell_classes(ncm, nsm, {0: OpSpec(op=N.OP_COLLIDE)}, s.e, solid)
+    flags = torch.tensor([c.flags for c in cc.table])[cc.cell_class.long()]
+    near = (flags & N.CLASS_NEAR_SOLID) != 0
+    assert int(near.sum()) == 26
+    assert bool(flags[2, 2, 2] & N.CLASS_SOLID)
