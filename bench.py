@@ -96,6 +96,28 @@ def cpu_baseline(name, budget_s=15.0):
                       f"({torch.get_num_threads()} threads)"}
 
 
+def copy_ceiling(nbytes, steps, warmup):
+    """Device-to-device copy of one population array (torch's copy kernel):
+    the achievable read + write bandwidth, and the reference kernel for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    src.uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    stop.record()
+    torch.cuda.synchronize()
+    s = start.elapsed_time(stop) / 1e3 / steps
+    return {"bytes_per_copy": 2 * src.numel() * 4, "ms_per_copy": s * 1e3,
+            "GBps": 2 * src.numel() * 4 / s / 1e9}
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(path):
@@ -112,6 +134,8 @@ def main():
     ap.add_argument("--workload", default="tgv3d_d3q19_bgk_256_f32",
                     choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy", action="store_true",
+                    help="only time a device copy of one population array")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -121,6 +145,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy:
+        stencil, _, storage, res, _, _ = WORKLOADS[args.workload]
+        q = {"D2Q9": 9, "D3Q19": 19, "D3Q27": 27}[stencil]
+        cells = int(np.prod(res if isinstance(res, list) else [res] * (2 if q == 9 else 3)))
+        print(json.dumps({"copy": copy_ceiling(q * cells * BYTES[storage], args.steps,
+                                               args.warmup)}), flush=True)
+        return
 
     import lettuce_amd as lt
     sim, S = build_sim(lt, args.workload)

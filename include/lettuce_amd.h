@@ -103,19 +103,24 @@ typedef struct lbm_cell_class {
   double u[3];        /* EQUILIBRIUM velocity, lattice components (x, y, z)    */
 } lbm_cell_class;
 
-/* learned-collision parameters (device pointers); LBM_COLL_LEARNED only */
+/* Learned collision (D2Q9).  With df = f - feq, ds the KBC shear part of df
+ * and dh = df - ds (which holds only the raw moments xxy, xyy, xxyy):
+ *   x   = (T, N, Pxy, xxy, xyy, xxyy)(df) / rho * in_scale        (bf16)
+ *   o   = W3 silu(W2 silu(W1 x + b1) + b2) + b3    (bf16 MFMA, FP32 sums)
+ *   r_g = 1/tau + o_g, clamped into [rate_lo, rate_hi]
+ *   f'  = f - ds/tau - sum_g r_g m_g(dh) phi_g
+ * A cell whose raw r_g is NaN or outside [stable_lo, stable_hi] takes the
+ * KBC update instead.  `packed` is the device copy of the
+ * LBM_LEARNED_PACKED_BYTES buffer written by lbm_learned_pack. */
+#define LBM_LEARNED_HIDDEN 32
+#define LBM_LEARNED_PACKED_BYTES 17408
 typedef struct lbm_learned {
-  const uint16_t* w1; /* bf16 [32][6]   first layer, row-major (out, in)      */
-  const float* b1;    /* [32]                                                  */
-  const uint16_t* w2; /* bf16 [32][32]                                         */
-  const float* b2;    /* [32]                                                  */
-  const uint16_t* w3; /* bf16 [3][32]                                          */
-  const float* b3;    /* [3]                                                   */
-  float in_scale;     /* inputs are non-equilibrium moments / rho * in_scale   */
-  float rate_lo;      /* predicted rates are clamped into [rate_lo, rate_hi]   */
+  const void* packed; /* device pointer, LBM_LEARNED_PACKED_BYTES, 16-byte aligned */
+  float in_scale;
+  float rate_lo;
   float rate_hi;
-  float stable_lo;    /* a raw rate outside [stable_lo, stable_hi] (or NaN)    */
-  float stable_hi;    /*   makes the cell fall back to KBC                     */
+  float stable_lo;
+  float stable_hi;
   int32_t pad;
 } lbm_learned;
 
@@ -222,6 +227,15 @@ int lbm_link_force_f64(int32_t stencil, int32_t storage, const int64_t n[3],
                        const int64_t stride[4], const void* f,
                        double rho0_shift, const int32_t* links, int64_t n_links,
                        double* out, void* workspace, void* stream);
+
+/* Host function: pack the MLP 6 -> 32 -> 32 -> 3 (row-major FP32 weights
+ * w1[32][6], w2[32][32], w3[3][32] and biases b1[32], b2[32], b3[3]) into the
+ * per-lane MFMA operand layout used by the learned-collision kernel; the
+ * weights are rounded to bf16 (round to nearest even).  `out` is host memory
+ * of LBM_LEARNED_PACKED_BYTES bytes; copy it to the device. */
+int lbm_learned_pack(const float* w1, const float* b1, const float* w2,
+                     const float* b2, const float* w3, const float* b3,
+                     void* out);
 
 #ifdef __cplusplus
 }

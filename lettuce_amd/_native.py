@@ -42,10 +42,12 @@ class LbmCellClass(ctypes.Structure):
                 ("rho", ctypes.c_double), ("u", ctypes.c_double * 3)]
 
 
+LEARNED_PACKED_BYTES = 17408
+LEARNED_HIDDEN = 32
+
+
 class LbmLearned(ctypes.Structure):
-    _fields_ = [("w1", ctypes.c_void_p), ("b1", ctypes.c_void_p),
-                ("w2", ctypes.c_void_p), ("b2", ctypes.c_void_p),
-                ("w3", ctypes.c_void_p), ("b3", ctypes.c_void_p),
+    _fields_ = [("packed", ctypes.c_void_p),
                 ("in_scale", ctypes.c_float), ("rate_lo", ctypes.c_float),
                 ("rate_hi", ctypes.c_float), ("stable_lo", ctypes.c_float),
                 ("stable_hi", ctypes.c_float), ("pad", ctypes.c_int32)]
@@ -89,6 +91,7 @@ _SIGNATURES = {
     "lbm_enstrophy_f64": ([_i32, _I64_3, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_link_force_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _i64,
                             _vp, _vp, _vp], ctypes.c_int),
+    "lbm_learned_pack": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
 }
 
 _lock = threading.Lock()

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <type_traits>
 
@@ -14,6 +15,8 @@ namespace lbm {
 KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q19(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
+int launch_learned(const KParams& p, const lbm_learned* l, int storage, bool mask, int mode,
+                   hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out);
 }  // namespace lbm
 
 using namespace lbm;
@@ -88,6 +91,23 @@ static int block_size_for(int64_t n2) {
   return 256;
 }
 
+// Block order for a grid of nb blocks: the XCD chunk order when the rows
+// split evenly over the 8 XCDs, else the XCD slab order, else storage order.
+// LETTUCE_AMD_BLOCK_ORDER=linear|slab|chunk overrides it where valid.
+static int block_order(const KParams& p, int64_t nb) {
+  const bool slab_ok = nb % 8 == 0 && nb >= 64;
+  const bool chunk_ok = slab_ok && p.n1 % 8 == 0;
+  const char* env = getenv("LETTUCE_AMD_BLOCK_ORDER");
+  if (env) {
+    if (!strcmp(env, "linear")) return ORDER_LINEAR;
+    if (!strcmp(env, "slab") && slab_ok) return ORDER_XCD_SLAB;
+    if (!strcmp(env, "chunk") && chunk_ok) return ORDER_XCD_CHUNK;
+  }
+  if (chunk_ok) return ORDER_XCD_CHUNK;
+  if (slab_ok) return ORDER_XCD_SLAB;
+  return ORDER_LINEAR;
+}
+
 static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
   if (!a) return fail(LBM_E_ARG, "NULL args");
   if (a->abi_version != LBM_ABI_VERSION)
@@ -108,15 +128,29 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
     return fail(LBM_E_ARG, "TRT needs tau > 0 and tau_minus > 0");
   if (a->collision == LBM_COLL_KBC && !(a->tau > 0)) return fail(LBM_E_ARG, "KBC needs tau > 0");
 
-  switch (a->stencil) {
-    case LBM_D2Q9: fn = pick_kernel_d2q9(a->storage, a->collision, mask, a->streaming); break;
-    case LBM_D3Q19: fn = pick_kernel_d3q19(a->storage, a->collision, mask, a->streaming); break;
-    case LBM_D3Q27: fn = pick_kernel_d3q27(a->storage, a->collision, mask, a->streaming); break;
-    default: fn = nullptr;
+  fn = nullptr;
+  if (a->collision == LBM_COLL_LEARNED) {
+    if (a->stencil != LBM_D2Q9) return fail(LBM_E_UNSUPPORTED, "the learned collision is D2Q9 only");
+    if (a->storage == LBM_F64)
+      return fail(LBM_E_UNSUPPORTED, "the learned collision computes in FP32; use F32 or shifted storage");
+    if (a->n[2] % 64 != 0)
+      return fail(LBM_E_UNSUPPORTED, "the learned collision needs the contiguous extent to be a multiple of 64");
+    if (!a->learned || !a->learned->packed) return fail(LBM_E_ARG, "learned parameters missing");
+    if (((uintptr_t)a->learned->packed) % 16 != 0) return fail(LBM_E_ARG, "packed parameters must be 16-byte aligned");
+    if (!(a->tau > 0.5)) return fail(LBM_E_ARG, "the learned collision needs tau > 1/2");
+    if (!(a->learned->rate_lo <= a->learned->rate_hi))
+      return fail(LBM_E_ARG, "rate_lo must not exceed rate_hi");
+  } else {
+    switch (a->stencil) {
+      case LBM_D2Q9: fn = pick_kernel_d2q9(a->storage, a->collision, mask, a->streaming); break;
+      case LBM_D3Q19: fn = pick_kernel_d3q19(a->storage, a->collision, mask, a->streaming); break;
+      case LBM_D3Q27: fn = pick_kernel_d3q27(a->storage, a->collision, mask, a->streaming); break;
+      default: fn = nullptr;
+    }
+    if (!fn)
+      return fail(LBM_E_UNSUPPORTED, "no native kernel for stencil %d, storage %d, collision %d",
+                  a->stencil, a->storage, a->collision);
   }
-  if (!fn)
-    return fail(LBM_E_UNSUPPORTED, "no native kernel for stencil %d, storage %d, collision %d",
-                a->stencil, a->storage, a->collision);
 
   p.fin = a->f_in;
   p.fout = a->f_out;
@@ -135,12 +169,13 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   const int64_t nb = (int64_t)p.nb2 * a->n[1] * a->n[0];
   if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");
   p.nblocks = (int)nb;
-  p.xcd_remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-  p.tau_inv = a->collision == LBM_COLL_BGK ? 1.0 / a->tau : 0.0;
+  p.order = block_order(p, nb);
+  const bool learned = a->collision == LBM_COLL_LEARNED;
+  p.tau_inv = (a->collision == LBM_COLL_BGK || learned) ? 1.0 / a->tau : 0.0;
   p.trt_p = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau) : 0.0;
   p.trt_m = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau_minus) : 0.0;
-  p.beta = a->collision == LBM_COLL_KBC ? 1.0 / (2.0 * a->tau) : 0.0;
-  p.beta_inv = a->collision == LBM_COLL_KBC ? 2.0 * a->tau : 0.0;
+  p.beta = (a->collision == LBM_COLL_KBC || learned) ? 1.0 / (2.0 * a->tau) : 0.0;
+  p.beta_inv = (a->collision == LBM_COLL_KBC || learned) ? 2.0 * a->tau : 0.0;
   p.rho0 = (a->storage == LBM_F16_SHIFTED || a->storage == LBM_BF16_SHIFTED) ? a->rho0_shift : 0.0;
   grid = dim3((unsigned)nb);
   block = dim3(bs);
@@ -163,25 +198,24 @@ int64_t lbm_struct_size(int which) {
 }
 
 int lbm_step(const lbm_step_args* a, void* stream) {
-  KParams p;
-  KernelFn fn;
-  dim3 grid, block;
-  int rc = prepare_step(a, p, fn, grid, block);
-  if (rc) return rc;
-  g_err[0] = 0;
-  hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
-  return check_launch("lbm_step");
+  return lbm_run(a, 1, stream);
 }
 
 int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   if (nsteps < 0) return fail(LBM_E_ARG, "negative step count");
-  if (nsteps == 0) return LBM_OK;
   KParams p;
   KernelFn fn;
   dim3 grid, block;
   int rc = prepare_step(a, p, fn, grid, block);
   if (rc) return rc;
+  if (nsteps == 0) return LBM_OK;
   g_err[0] = 0;
+  if (a->collision == LBM_COLL_LEARNED) {
+    rc = launch_learned(p, a->learned, a->storage, a->cell_class != nullptr, a->streaming,
+                        (hipStream_t)stream, nsteps, a->f_in, a->f_out);
+    if (rc) return fail(rc, "no learned kernel for storage %d", a->storage);
+    return check_launch("lbm_run (learned)");
+  }
   const void* bufs[2] = {a->f_in, a->f_out};
   for (int64_t k = 0; k < nsteps; ++k) {
     p.fin = bufs[k & 1];

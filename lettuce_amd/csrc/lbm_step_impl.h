@@ -30,10 +30,37 @@ struct KParams {
   long long ncells;   // n0*n1*n2 (bc_field plane stride)
   int nb2;            // blocks along i2
   int nblocks;        // total blocks
-  int xcd_remap;      // 1: give each XCD a contiguous slab of blocks
+  int order;          // block order, one of BlockOrder
   double tau_inv, trt_p, trt_m, beta, beta_inv;
   double rho0;
 };
+
+// Which row (i0, i1) and which run b2 of the row a block takes.  The
+// hardware hands block b to XCD b % 8, and each XCD has its own L2:
+//  LINEAR     rows in storage order (i1 fastest);
+//  XCD_SLAB   XCD k sweeps the k-th eighth of the blocks in storage order,
+//             so its i1 +- 1 neighbour rows stay in its L2;
+//  XCD_CHUNK  XCD k owns rows i1 in the k-th eighth of [0, n1) and walks
+//             them with i0 slowest, so both the i1 +- 1 rows and the
+//             i0 +- 1 planes of its chunk were read moments before.
+enum BlockOrder { ORDER_LINEAR = 0, ORDER_XCD_SLAB = 1, ORDER_XCD_CHUNK = 2 };
+
+__device__ __forceinline__ void block_tile(const KParams& p, int b, int& i0, int& i1, int& b2) {
+  if (p.order == ORDER_XCD_CHUNK) {
+    const int k = b & 7, j = b >> 3;
+    const int h = p.n1 >> 3;  // rows per XCD chunk
+    b2 = j % p.nb2;
+    const int t = j / p.nb2;
+    i1 = k * h + t % h;
+    i0 = t / h;
+    return;
+  }
+  if (p.order == ORDER_XCD_SLAB) b = (b & 7) * (p.nblocks >> 3) + (b >> 3);
+  b2 = b % p.nb2;
+  const int r = b / p.nb2;
+  i1 = r % p.n1;
+  i0 = r / p.n1;
+}
 
 template <class C>
 __device__ __forceinline__ CollParams<C> coll_params(const KParams& p) {
@@ -119,47 +146,14 @@ __device__ __forceinline__ void collide(C (&f)[S::Q], const CollParams<C>& cp) {
   // LBM_COLL_NONE: f unchanged
 }
 
-template <class S, class ST, int COLL, bool MASK, int MODE>
-__global__ void __launch_bounds__(256) step_kernel(const KParams p) {
+// The operator of a non-collision cell (one of LBM_OP_BOUNCE_BACK,
+// LBM_OP_EQUILIBRIUM, LBM_OP_OUTLET, LBM_OP_KEEP) on its populations f.
+template <class S, class ST, bool PULL>
+__device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op, uint32_t flags,
+                                                  int cell, int i0, int i1, int i2,
+                                                  typename ST::C (&f)[S::Q]) {
   using C = typename ST::C;
-  using T = typename ST::T;
-  constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
-  constexpr bool PUSH = (MODE & LBM_POST_STREAMING) != 0;
-
-  // XCD-aware block order: hardware places block b on XCD b % 8, so block b
-  // takes logical tile (b % 8) * (nblocks / 8) + b / 8 and each XCD sweeps a
-  // contiguous slab whose halo rows stay in its own L2.
-  int b = blockIdx.x;
-  if (p.xcd_remap) b = (b & 7) * (p.nblocks >> 3) + (b >> 3);
-  const int b2 = b % p.nb2;
-  const int r = b / p.nb2;
-  const int i1 = r % p.n1;
-  const int i0 = r / p.n1;
-  const int i2 = b2 * blockDim.x + threadIdx.x;
-  if (i2 >= p.n2) return;
-
-  int a0[3], a1[3], a2[3];
-  axis_offsets(i0, p.n0, p.s0, a0);
-  axis_offsets(i1, p.n1, p.s1, a1);
-  axis_offsets(i2, p.n2, 1, a2);
-  const int cell = a0[1] + a1[1] + a2[1];
-
-  uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
-  if (MASK) {
-    const lbm_cell_class& K = p.ct[p.cls[cell]];
-    nsm = K.no_stream;
-    flags = K.flags;
-    op = K.op;
-  }
-
-  C f[S::Q];
-  if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
-  else gather_own<S, ST>(p, cell, f);
-
-  const CollParams<C> cp = coll_params<C>(p);
-  if (!MASK || op == LBM_OP_COLLIDE) {
-    collide<S, C, COLL>(f, cp);
-  } else if (op == LBM_OP_BOUNCE_BACK) {
+  if (op == LBM_OP_BOUNCE_BACK) {
     // fullway bounce-back: f_q <- f_opp(q) (bounce_back_boundary.py:17-18)
     C g[S::Q];
 #pragma unroll
@@ -207,7 +201,16 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
     equilibrium<S, C>((C)K.rho, u, f);
   }
   // LBM_OP_KEEP: unchanged
+}
 
+// Write the cell's post-collision populations: in place (no post-streaming)
+// or pushed to the neighbours with the streaming rules of the destination.
+template <class S, class ST, bool MASK, bool PUSH>
+__device__ __forceinline__ void store_cell(const KParams& p, int cell, const int (&a0)[3],
+                                           const int (&a1)[3], const int (&a2)[3], uint32_t nsm,
+                                           uint32_t flags, const typename ST::C (&f)[S::Q]) {
+  using C = typename ST::C;
+  using T = typename ST::T;
   T* fout = (T*)p.fout;
   if (!PUSH) {
 #pragma unroll
@@ -241,6 +244,43 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
     }
   }
 }
+
+template <class S, class ST, int COLL, bool MASK, int MODE>
+__global__ void __launch_bounds__(256) step_kernel(const KParams p) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
+  constexpr bool PUSH = (MODE & LBM_POST_STREAMING) != 0;
+
+  int i0, i1, b2;
+  block_tile(p, blockIdx.x, i0, i1, b2);
+  const int i2 = b2 * blockDim.x + threadIdx.x;
+  if (i2 >= p.n2) return;
+
+  int a0[3], a1[3], a2[3];
+  axis_offsets(i0, p.n0, p.s0, a0);
+  axis_offsets(i1, p.n1, p.s1, a1);
+  axis_offsets(i2, p.n2, 1, a2);
+  const int cell = a0[1] + a1[1] + a2[1];
+
+  uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
+  if (MASK) {
+    const lbm_cell_class& K = p.ct[p.cls[cell]];
+    nsm = K.no_stream;
+    flags = K.flags;
+    op = K.op;
+  }
+
+  C f[S::Q];
+  if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
+  else gather_own<S, ST>(p, cell, f);
+
+  const CollParams<C> cp = coll_params<C>(p);
+  if (!MASK || op == LBM_OP_COLLIDE) collide<S, C, COLL>(f, cp);
+  else apply_boundary_op<S, ST, PULL>(p, op, flags, cell, i0, i1, i2, f);
+  store_cell<S, ST, MASK, PUSH>(p, cell, a0, a1, a2, nsm, flags, f);
+}
+
 
 // ---------------------------------------------------------------------------
 // dispatch

@@ -1,4 +1,5 @@
 """Operators, flows and reporters (lettuce.ext counterparts)."""
 from ._operators import *  # noqa: F401,F403
+from ._learned import *  # noqa: F401,F403
 from ._flows import *  # noqa: F401,F403
 from ._reporter import *  # noqa: F401,F403

@@ -759,3 +759,138 @@ def mass(flow, no_mass_mask=None):
     if no_mass_mask is not None:
         m -= (f * no_mass_mask.to(dtype=f.dtype)).sum()
     return m
+
+
+# --------------------------------------------------------------------------
+# learned collision (no lettuce counterpart; nearest hook MRTCollision,
+# lettuce/ext/_collision/mrt_collision.py:8-35, KBC decomposition of
+# kbc_collision.py:76-160)
+# --------------------------------------------------------------------------
+def bf16_round(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+# D2Q9 dual vectors of the three higher-order raw moments of a population
+# vector with zero density, momentum and second moments:
+# m_xxy -> PHI_XXY, m_xyy -> PHI_XYY, m_xxyy -> PHI_XXYY
+PHI_XXY = [0.0, 0.0, -0.5, 0.0, 0.5, 0.25, 0.25, -0.25, -0.25]
+PHI_XYY = [0.0, -0.5, 0.0, 0.5, 0.0, 0.25, -0.25, -0.25, 0.25]
+PHI_XXYY = [1.0, -0.5, -0.5, -0.5, -0.5, 0.25, 0.25, 0.25, 0.25]
+
+
+def d2q9_moment_rows(dtype=torch.float64):
+    """rows of the raw moments T = xx + yy, N = xx - yy, Pxy, xxy, xyy, xxyy"""
+    e = torch.tensor(D2Q9.e, dtype=dtype)
+    x, y = e[:, 0], e[:, 1]
+    return torch.stack([x * x + y * y, x * x - y * y, x * y, x * x * y, x * y * y,
+                        x * x * y * y])
+
+
+class LearnedParameters:
+    """MLP 6 -> 32 -> 32 -> 3 with SiLU; weights stored in bf16."""
+
+    def __init__(self, w1, b1, w2, b2, w3, b3, in_scale=100.0, rate_lo=0.05,
+                 rate_hi=1.95, stable_lo=0.0, stable_hi=2.0):
+        self.w1 = bf16_round(torch.as_tensor(w1, dtype=torch.float32))
+        self.w2 = bf16_round(torch.as_tensor(w2, dtype=torch.float32))
+        self.w3 = bf16_round(torch.as_tensor(w3, dtype=torch.float32))
+        self.b1 = torch.as_tensor(b1, dtype=torch.float32)
+        self.b2 = torch.as_tensor(b2, dtype=torch.float32)
+        self.b3 = torch.as_tensor(b3, dtype=torch.float32)
+        self.in_scale = float(in_scale)
+        self.rate_lo, self.rate_hi = float(rate_lo), float(rate_hi)
+        self.stable_lo, self.stable_hi = float(stable_lo), float(stable_hi)
+
+    @staticmethod
+    def seeded(seed=0, hidden=32, out_scale=0.1):
+        """torch.manual_seed(seed) + nn.Linear default init (U(+-1/sqrt(fan_in)))
+        with the last layer scaled by out_scale."""
+        g = torch.Generator().manual_seed(seed)
+
+        def lin(o, i):
+            bound = 1.0 / np.sqrt(i)
+            w = (torch.rand(o, i, generator=g) * 2 - 1) * bound
+            b = (torch.rand(o, generator=g) * 2 - 1) * bound
+            return w, b
+        w1, b1 = lin(hidden, 6)
+        w2, b2 = lin(hidden, hidden)
+        w3, b3 = lin(3, hidden)
+        return LearnedParameters(w1, b1, w2, b2, w3 * out_scale, b3 * out_scale)
+
+
+class LearnedCollision:
+    """D2Q9 learned collision.
+
+    With df = f - feq, ds the KBC shear part of df and dh = df - ds (which
+    carries only the moments xxy, xyy, xxyy):
+        x   = (T, N, Pxy, xxy, xyy, xxyy)(df) / rho * in_scale   (bf16)
+        o   = W3 silu(W2 silu(W1 x + b1) + b2) + b3   (bf16 operands, FP32 sums)
+        r_g = 1/tau + o_g                             raw rates of xxy, xyy, xxyy
+        f'  = f - ds / tau - sum_g clamp(r_g) m_g(dh) phi_g
+    A cell whose raw rates are NaN or leave [stable_lo, stable_hi] relaxes
+    with KBC instead.  With every rate equal to beta*gamma the update is KBC,
+    with every rate 1/tau it is BGK.
+    """
+
+    def __init__(self, params: LearnedParameters):
+        self.params = params
+        self.kbc = KBCCollision()
+        self.force_rates = None   # test hook: fixed rates [3] or [3, *res]
+
+    def mlp(self, x):
+        p = self.params
+        h = torch.einsum("oi,i...->o...", p.w1, bf16_round(x)) + p.b1.reshape(
+            [-1] + [1] * (x.dim() - 1))
+        h = torch.nn.functional.silu(h)
+        h = torch.einsum("oi,i...->o...", p.w2, bf16_round(h)) + p.b2.reshape(
+            [-1] + [1] * (x.dim() - 1))
+        h = torch.nn.functional.silu(h)
+        return torch.einsum("oi,i...->o...", p.w3, bf16_round(h)) + p.b3.reshape(
+            [-1] + [1] * (x.dim() - 1))
+
+    def inputs(self, flow, df, rho):
+        rows = d2q9_moment_rows(df.dtype)
+        m = torch.einsum("kq,q...->k...", rows, df)
+        return (m / rho * self.params.in_scale).to(torch.float32)
+
+    def rates(self, flow):
+        """(clamped rates [3, *res], fallback mask [*res])"""
+        tau = flow.units.relaxation_parameter_lu
+        feq = flow.equilibrium(flow)
+        df = flow.f - feq
+        x = self.inputs(flow, df, flow.rho())
+        raw = (1.0 / tau + self.mlp(x)).to(flow.f.dtype)
+        if self.force_rates is not None:
+            raw = torch.as_tensor(self.force_rates, dtype=flow.f.dtype)
+            raw = raw.reshape([3] + [1] * flow.stencil.d).expand([3] + list(df.shape[1:]))
+        p = self.params
+        fallback = (torch.isnan(raw) | (raw < p.stable_lo) | (raw > p.stable_hi)).any(0)
+        return raw.clamp(p.rate_lo, p.rate_hi), fallback
+
+    def __call__(self, flow):
+        assert isinstance(flow.stencil, D2Q9)
+        tau = flow.units.relaxation_parameter_lu
+        feq = flow.equilibrium(flow)
+        df = flow.f - feq
+        dtype = flow.f.dtype
+        e = torch.tensor(D2Q9.e, dtype=dtype)
+        xx = torch.einsum("q,q...->...", e[:, 0] ** 2, df)
+        yy = torch.einsum("q,q...->...", e[:, 1] ** 2, df)
+        xy = torch.einsum("q,q...->...", e[:, 0] * e[:, 1], df)
+        T, N = xx + yy, xx - yy
+        ds = torch.zeros_like(df)
+        ds[0] = -T
+        ds[1] = ds[3] = 0.25 * (T + N)
+        ds[2] = ds[4] = 0.25 * (T - N)
+        ds[5] = ds[7] = 0.25 * xy
+        ds[6] = ds[8] = -0.25 * xy
+        dh = df - ds
+        rows = d2q9_moment_rows(dtype)[3:]
+        mg = torch.einsum("kq,q...->k...", rows, dh)
+        phi = torch.tensor([PHI_XXY, PHI_XYY, PHI_XXYY], dtype=dtype)
+        omega, fallback = self.rates(flow)
+        upd = torch.einsum("kq,k...->q...", phi, omega * mg)
+        f_learned = flow.f - ds / tau - upd
+        if not fallback.any():
+            return f_learned
+        return torch.where(fallback[None], self.kbc(flow), f_learned)

@@ -1,0 +1,101 @@
+"""GPU parity of the learned collision (csrc/lbm_learned.hip) against the
+oracle's LearnedCollision, plus its two reductions: an MLP that outputs 0
+gives BGK, and rates outside the stable range give KBC."""
+import numpy as np
+import pytest
+import torch
+
+import lettuce_amd as lt
+from oracle import lettuce_torch as O
+from tests.test_parity_gpu import ArrayFlow, ctx, rel_err, F32, F64
+from tests.test_oracle import _Grid, _perturbed
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(seed=0, out_scale=1.0, **kw):
+    p = lt.LearnedParameters.seeded(seed, out_scale=out_scale, **kw)
+    op = O.LearnedParameters(*[t.clone() for t in p.tensors()], in_scale=p.in_scale,
+                             rate_lo=p.rate_lo, rate_hi=p.rate_hi,
+                             stable_lo=p.stable_lo, stable_hi=p.stable_hi)
+    return p, op
+
+
+def zero_params(**kw):
+    H = 32
+    z = torch.zeros
+    return lt.LearnedParameters(z(H, 6), z(H), z(H, H), z(H), z(3, H), z(3), **kw)
+
+
+def test_packing_roundtrip_shapes():
+    p, _ = pair()
+    buf = p.packed(torch.device("cuda:0"))
+    assert buf.numel() == 17408 and buf.data_ptr() % 16 == 0
+
+
+def test_zero_mlp_is_bgk():
+    """o = 0 gives every higher-order rate 1/tau: the update is BGK."""
+    f0 = _perturbed(O.D2Q9(), 64, F32, seed=3).f.numpy()
+    flow = ArrayFlow(ctx(F32), f0, lt.D2Q9())
+    tau = flow.units.relaxation_parameter_lu
+    out = lt.LearnedCollision(zero_params())(flow).cpu().numpy()
+    ref = lt.BGKCollision(tau)(flow).cpu().numpy()
+    assert rel_err(out, ref) < 2e-6
+
+
+def test_unstable_rates_fall_back_to_kbc():
+    p = zero_params()
+    p.b3 = torch.full((3,), 5.0)  # raw rates 1/tau + 5 > stable_hi
+    f0 = _perturbed(O.D2Q9(), 64, F32, seed=4).f.numpy()
+    flow = ArrayFlow(ctx(F32), f0, lt.D2Q9())
+    out = lt.LearnedCollision(p)(flow).cpu().numpy()
+    ref = lt.KBCCollision()(flow).cpu().numpy()
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_learned_collision_vs_oracle(seed):
+    p, op = pair(seed)
+    oflow = _perturbed(O.D2Q9(), 64, F32, seed=5 + seed)
+    f0 = oflow.f.numpy().copy()
+    ref_coll = O.LearnedCollision(op)
+    ref = ref_coll(oflow).numpy()
+    _, fb = ref_coll.rates(oflow)
+    flow = ArrayFlow(ctx(F32), f0, lt.D2Q9())
+    out = lt.LearnedCollision(p)(flow).cpu().numpy()
+    assert rel_err(out, ref) <= 1e-5
+    print("fallback fraction", float(fb.float().mean()))
+
+
+def test_learned_tgv_100_steps_vs_oracle():
+    p, op = pair(0, out_scale=0.1)
+    oflow = O.TaylorGreenVortex(O.Context("cpu", F32), 64, 300, 0.1, O.D2Q9())
+    f0 = oflow.f.numpy().copy()
+    O.Simulation(oflow, O.LearnedCollision(op), [], O.StreamingStrategy.PRE_STREAMING)(100)
+    flow = lt.TaylorGreenVortex(ctx(F32), 64, 300, 0.1, lt.D2Q9(), initialize_fneq=False)
+    flow.f = torch.from_numpy(f0)
+    lt.Simulation(flow, lt.LearnedCollision(p), [], lt.StreamingStrategy.PRE_STREAMING)(100)
+    assert rel_err(flow.f.cpu().numpy(), oflow.f.numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16_shifted"])
+def test_learned_channel_cylinder_vs_oracle(storage):
+    """halfway bounce-back, equilibrium inlet and outlet around the learned
+    collision; FP32 storage against the oracle, shifted FP16 storage
+    against the FP32 native run."""
+    p, op = pair(0, out_scale=0.1)
+    nx, ny = 128, 64
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+    mask = (xx - 32) ** 2 + (yy - 31.5) ** 2 < 8.0 ** 2
+    oflow = O.ObstacleChannel(O.Context("cpu", F32), [nx, ny], 100, 0.1,
+                              domain_length_x=nx / 16.0, mask=mask, stencil=O.D2Q9())
+    f0 = oflow.f.numpy().copy()
+    O.Simulation(oflow, O.LearnedCollision(op), [], O.StreamingStrategy.PRE_STREAMING)(100)
+    c = ctx(F32, storage)
+    flow = lt.Obstacle(c, [nx, ny], 100, 0.1, domain_length_x=nx / 16.0, stencil=lt.D2Q9(),
+                       outlet="equilibrium", solid="halfway", walls=True)
+    flow.mask = mask
+    flow.f = torch.from_numpy(f0)
+    lt.Simulation(flow, lt.LearnedCollision(p), [], lt.StreamingStrategy.PRE_STREAMING)(100)
+    err = rel_err(flow.f.cpu().numpy(), oflow.f.numpy())
+    assert err <= (1e-5 if storage == "f32" else 2e-3)

@@ -137,8 +137,10 @@ def test_tgv2d_native_flow_setup_matches_lettuce():
 
 @pytest.mark.parametrize("name,n", [("D3Q19", 32), ("D3Q27", 16)])
 def test_tgv3d_bgk_1000_steps_vs_oracle_fp32(name, n):
+    """Re 100 keeps BGK stable at these small grids (at Re 1600, tau - 1/2
+    is 5e-4 and both sides diverge)."""
     s, os_ = _stencils()[name]
-    oflow = O.TaylorGreenVortex(O.Context("cpu", F32), n, 1600, 0.1, os_)
+    oflow = O.TaylorGreenVortex(O.Context("cpu", F32), n, 100, 0.1, os_)
     f0 = oflow.f.numpy().copy()
     tau = oflow.units.relaxation_parameter_lu
     O.Simulation(oflow, O.BGKCollision(tau), [], O.StreamingStrategy.PRE_STREAMING)(1000)
@@ -154,7 +156,7 @@ def test_tgv3d_golden_10_steps():
         flow = ArrayFlow(ctx(F32), g[f"{name}_f0"], s)
         out = native_run(flow, lt.BGKCollision(float(g[f"{name}_tau"])), 10,
                          lt.StreamingStrategy.PRE_STREAMING)
-        assert rel_err(out, g[f"{name}_f10"]) <= 1e-6
+        assert rel_err(out, g[f"{name}_f10"]) <= REL32
 
 
 # --------------------------------------------------------------- TRT / KBC
@@ -184,16 +186,21 @@ def test_kbc_matches_lettuce(tag, dtype):
 
 
 @pytest.mark.parametrize("name,n", [("D2Q9", 64), ("D3Q27", 12)])
-def test_kbc_tgv_vs_oracle_fp32(name, n):
+def test_kbc_tgv_vs_oracle_fp64(name, n):
+    """300 KBC steps of a TGV in FP64.  KBC's stabiliser gamma is a ratio of
+    sums of the higher-order part dh, which starts at rounding level, and it
+    jumps to 2 below 1e-15 (kbc_collision.py:150-157); long FP32 runs from an
+    equilibrium state therefore follow rounding noise and are compared in
+    FP64 here (FP32 parity is pinned on well-conditioned states above)."""
     s, os_ = _stencils()[name]
-    oflow = O.TaylorGreenVortex(O.Context("cpu", F32), n, 3000, 0.1, os_)
+    oflow = O.TaylorGreenVortex(O.Context("cpu", F64), n, 3000, 0.1, os_)
     f0 = oflow.f.numpy().copy()
     steps = 300
     O.Simulation(oflow, O.KBCCollision(), [], O.StreamingStrategy.PRE_STREAMING)(steps)
-    flow = lt.TaylorGreenVortex(ctx(F32), n, 3000, 0.1, s, initialize_fneq=False)
+    flow = lt.TaylorGreenVortex(ctx(F64), n, 3000, 0.1, s, initialize_fneq=False)
     flow.f = torch.from_numpy(f0)
     out = native_run(flow, lt.KBCCollision(), steps, lt.StreamingStrategy.PRE_STREAMING)
-    assert rel_err(out, oflow.f.numpy()) <= REL32
+    assert rel_err(out, oflow.f.numpy()) <= 1e-9
 
 
 @pytest.mark.parametrize("name", ["D2Q9", "D3Q19", "D3Q27"])
@@ -308,7 +315,9 @@ def test_halfway_channel_cylinder_vs_oracle(dtype, strategy):
         d = hw_links[:, 1].long()
         vals = fpost.reshape(9, -1)[d, cells]
         ref = (2 * vals[:, None] * e[d]).sum(0).numpy()
-        assert np.allclose(F, ref, rtol=1e-9 if dtype == F64 else 1e-5, atol=1e-12)
+        scale = np.abs(ref).max()
+        tol = 1e-9 if dtype == F64 else 1e-5
+        assert np.all(np.abs(F - ref) <= tol * scale)
 
 
 # ---------------------------------------------------------- shifted storage
@@ -332,8 +341,9 @@ def test_shifted_storage_one_step_rounding(storage, tdt):
     out = native_run(flow, lt.BGKCollision(tau), 1, lt.StreamingStrategy.PRE_STREAMING)
     dev = np.abs(ref - w.numpy())
     ulp = np.maximum(dev, 1e-30) * (2.0 ** -10 if tdt == torch.float16 else 2.0 ** -7)
-    floor = 2.0 ** -24 if tdt == torch.float16 else 0.0   # fp16 subnormal spacing
-    assert np.all(np.abs(out - ref) <= ulp + floor + 1e-12)
+    # plus two FP32 ulps of f ~ 0.4: the arithmetic before the rounding
+    floor = 6e-8 + (2.0 ** -24 if tdt == torch.float16 else 0.0)
+    assert np.all(np.abs(out - ref) <= ulp + floor)
 
 
 @pytest.mark.parametrize("storage,bound", [("f16_shifted", 2e-3), ("bf16_shifted", 2e-2)])
