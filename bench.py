@@ -97,25 +97,41 @@ def cpu_baseline(name, budget_s=15.0):
 
 
 def copy_ceiling(nbytes, steps, warmup):
-    """Device-to-device copy of one population array (torch's copy kernel):
-    the achievable read + write bandwidth, and the reference kernel for
-    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """Device-to-device copies of one population array: the HIP runtime's
+    copy (torch's copy_, the reference kernel for scaling the FETCH_SIZE /
+    WRITE_SIZE counters) and lbm_stream_copy (16 B per lane, plain and
+    non-temporal stores).  GB/s counts read + write bytes."""
+    import ctypes
+    from lettuce_amd import _native as N
+    nbytes -= nbytes % 16
     src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
     src.uniform_()
     dst = torch.empty_like(src)
-    for _ in range(warmup):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(steps):
-        dst.copy_(src)
-    stop.record()
-    torch.cuda.synchronize()
-    s = start.elapsed_time(stop) / 1e3 / steps
-    return {"bytes_per_copy": 2 * src.numel() * 4, "ms_per_copy": s * 1e3,
-            "GBps": 2 * src.numel() * 4 / s / 1e9}
+    stream = N.stream_handle(src.device)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            fn()
+        stop.record()
+        torch.cuda.synchronize()
+        s = start.elapsed_time(stop) / 1e3 / steps
+        return round(2 * nbytes / s / 1e9, 1)
+
+    def ours(nt):
+        return lambda: N.check(N.lib().lbm_stream_copy(
+            ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), nbytes, nt,
+            stream), "lbm_stream_copy")
+    out = {"bytes_per_copy": 2 * nbytes, "runtime_copy_GBps": timed(lambda: dst.copy_(src)),
+           "stream_copy_GBps": timed(ours(0)), "stream_copy_nt_GBps": timed(ours(1))}
+    out["best_GBps"] = max(out["runtime_copy_GBps"], out["stream_copy_GBps"],
+                           out["stream_copy_nt_GBps"])
+    return out
 
 
 def load_traffic(name):
@@ -188,6 +204,9 @@ def main():
                      "bytes_per_cell": bytes_per_cell},
         "wall_s": round(wall_s, 4),
     }
+    ceiling = copy_ceiling(S.q * cells * BYTES[storage], 20, 3)
+    line["roofline"]["measured_copy_ceiling"] = ceiling["best_GBps"]
+    line["roofline"]["frac_of_copy_ceiling"] = round(achieved / ceiling["best_GBps"], 4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line), flush=True)

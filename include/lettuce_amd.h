@@ -228,6 +228,13 @@ int lbm_link_force_f64(int32_t stencil, int32_t storage, const int64_t n[3],
                        double rho0_shift, const int32_t* links, int64_t n_links,
                        double* out, void* workspace, void* stream);
 
+/* Bandwidth probe with no lettuce counterpart: dst = src for `bytes` bytes
+ * (a multiple of 16, both pointers 16-byte aligned), 16 bytes per lane,
+ * optionally with non-temporal stores.  It measures the read + write HBM
+ * ceiling that the step kernels are compared with. */
+int lbm_stream_copy(const void* src, void* dst, int64_t bytes,
+                    int32_t nontemporal, void* stream);
+
 /* Host function: pack the MLP 6 -> 32 -> 32 -> 3 (row-major FP32 weights
  * w1[32][6], w2[32][32], w3[3][32] and biases b1[32], b2[32], b3[3]) into the
  * per-lane MFMA operand layout used by the learned-collision kernel; the

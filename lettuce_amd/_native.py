@@ -92,6 +92,7 @@ _SIGNATURES = {
     "lbm_link_force_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _i64,
                             _vp, _vp, _vp], ctypes.c_int),
     "lbm_learned_pack": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_stream_copy": ([_vp, _vp, _i64, _i32, _vp], ctypes.c_int),
 }
 
 _lock = threading.Lock()

@@ -17,7 +17,18 @@ KernelFn pick_kernel_d3q19(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
 int launch_learned(const KParams& p, const lbm_learned* l, int storage, bool mask, int mode,
                    hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out);
+KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt);
+KernelFn pick_pair_kernel_d3q19(int storage, int coll, int mode, bool nt);
+KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
 }  // namespace lbm
+
+// Launch tuning read from the environment (for measurements):
+//  LETTUCE_AMD_CELLS_PER_THREAD=1|2  pair kernel where available (default 2)
+//  LETTUCE_AMD_NT_STORE=0|1          non-temporal stores in the pair kernel (default 1)
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
 
 using namespace lbm;
 
@@ -99,12 +110,11 @@ static int block_order(const KParams& p, int64_t nb) {
   const bool chunk_ok = slab_ok && p.n1 % 8 == 0;
   const char* env = getenv("LETTUCE_AMD_BLOCK_ORDER");
   if (env) {
-    if (!strcmp(env, "linear")) return ORDER_LINEAR;
     if (!strcmp(env, "slab") && slab_ok) return ORDER_XCD_SLAB;
     if (!strcmp(env, "chunk") && chunk_ok) return ORDER_XCD_CHUNK;
   }
-  if (chunk_ok) return ORDER_XCD_CHUNK;
-  if (slab_ok) return ORDER_XCD_SLAB;
+  // measured on MI355X (D3Q19 256^3 FP32): storage order 5.69 TB/s, XCD
+  // chunks 5.57 TB/s, XCD slabs 4.76 TB/s
   return ORDER_LINEAR;
 }
 
@@ -164,8 +174,23 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.s1 = (int)a->stride[2];
   p.sq = a->stride[0];
   p.ncells = a->n[0] * a->n[1] * a->n[2];
-  const int bs = block_size_for(a->n[2]);
-  p.nb2 = (int)((a->n[2] + bs - 1) / bs);
+  // two cells per thread where a pair kernel exists and rows have even length
+  int64_t row_threads = a->n[2];
+  if (fn && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 2) == 2) {
+    const bool nt = env_int("LETTUCE_AMD_NT_STORE", 1) != 0;
+    KernelFn pf = nullptr;
+    switch (a->stencil) {
+      case LBM_D2Q9: pf = pick_pair_kernel_d2q9(a->storage, a->collision, a->streaming, nt); break;
+      case LBM_D3Q19: pf = pick_pair_kernel_d3q19(a->storage, a->collision, a->streaming, nt); break;
+      case LBM_D3Q27: pf = pick_pair_kernel_d3q27(a->storage, a->collision, a->streaming, nt); break;
+    }
+    if (pf) {
+      fn = pf;
+      row_threads = a->n[2] / 2;
+    }
+  }
+  const int bs = block_size_for(row_threads);
+  p.nb2 = (int)((row_threads + bs - 1) / bs);
   const int64_t nb = (int64_t)p.nb2 * a->n[1] * a->n[0];
   if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");
   p.nblocks = (int)nb;
@@ -660,3 +685,35 @@ int lbm_link_force_f64(int32_t stencil, int32_t storage, const int64_t n[3], con
 }
 
 }  // extern "C"
+
+namespace {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__global__ void __launch_bounds__(256) k_stream_copy(const u32x4* __restrict__ src,
+                                                     u32x4* __restrict__ dst, int64_t n16) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n16) return;
+  const u32x4 v = src[i];
+  if (NT) __builtin_nontemporal_store(v, dst + i);
+  else dst[i] = v;
+}
+}  // namespace
+
+extern "C" int lbm_stream_copy(const void* src, void* dst, int64_t bytes, int32_t nontemporal,
+                               void* stream) {
+  if (!src || !dst) return fail(LBM_E_ARG, "NULL pointer");
+  if (bytes < 0 || bytes % 16 != 0) return fail(LBM_E_ARG, "bytes must be a non-negative multiple of 16");
+  if (((uintptr_t)src | (uintptr_t)dst) % 16 != 0) return fail(LBM_E_ARG, "pointers must be 16-byte aligned");
+  const int64_t n16 = bytes / 16;
+  if (n16 == 0) return LBM_OK;
+  const int64_t blocks = (n16 + 255) / 256;
+  if (blocks > 0x7fffffff) return fail(LBM_E_ARG, "copy too large");
+  g_err[0] = 0;
+  if (nontemporal)
+    hipLaunchKernelGGL(k_stream_copy<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const u32x4*)src, (u32x4*)dst, n16);
+  else
+    hipLaunchKernelGGL(k_stream_copy<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const u32x4*)src, (u32x4*)dst, n16);
+  return check_launch("lbm_stream_copy");
+}

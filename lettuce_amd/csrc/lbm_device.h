@@ -86,10 +86,14 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// dec: stored value -> population, enc: population -> stored value;
+// ld / st: one element, stores bypassing the caches (streamed output)
 struct StF32 {
   using T = float;
   using C = float;
   static constexpr bool shifted = false;
+  __device__ static C dec(T x, C) { return x; }
+  __device__ static T enc(C v, C) { return v; }
   __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
   __device__ static void st(T* p, uint32_t i, C v, C) {
     __builtin_nontemporal_store(v, p + i);
@@ -99,6 +103,8 @@ struct StF64 {
   using T = double;
   using C = double;
   static constexpr bool shifted = false;
+  __device__ static C dec(T x, C) { return x; }
+  __device__ static T enc(C v, C) { return v; }
   __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
   __device__ static void st(T* p, uint32_t i, C v, C) {
     __builtin_nontemporal_store(v, p + i);
@@ -108,6 +114,8 @@ struct StF16 {
   using T = _Float16;
   using C = float;
   static constexpr bool shifted = true;
+  __device__ static C dec(T x, C sh) { return (float)x + sh; }
+  __device__ static T enc(C v, C sh) { return (_Float16)(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
     return (float)p[i] + sh;
   }
@@ -119,6 +127,8 @@ struct StBF16 {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
+  __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
+  __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
     return bf16_to_f32(p[i]) + sh;
   }

@@ -282,10 +282,111 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
 }
 
 
+// Unmasked pull (or no-streaming) step with two neighbouring cells per
+// thread along i2: every population moves as one 2-element load and one
+// 2-element store (global_load_dwordx2 for FP32).  A pull along i2 reads the
+// pair shifted by one element (4-byte aligned for FP32, which gfx950 loads
+// directly); only the pair that wraps around the periodic row end reads two
+// single elements.  NT selects streaming (non-temporal) stores.
+template <class S, class ST, int COLL, bool PULL, bool NT>
+__global__ void __launch_bounds__(256) pair_kernel(const KParams p) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  typedef T T2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+  int i0, i1, b2;
+  block_tile(p, blockIdx.x, i0, i1, b2);
+  const int z0 = 2 * (b2 * (int)blockDim.x + (int)threadIdx.x);
+  if (z0 >= p.n2) return;
+  int a0[3], a1[3];
+  axis_offsets(i0, p.n0, p.s0, a0);
+  axis_offsets(i1, p.n1, p.s1, a1);
+  const bool lo_edge = z0 == 0, hi_edge = z0 + 2 == p.n2;
+  const T* fin = (const T*)p.fin;
+  C f0[S::Q], f1[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C sh = shift_of<ST>(S::w[q], p);
+    const int ez = PULL ? go<S>(q, 2) : 0;
+    const T* row = fin + q * p.sq + (PULL ? a0[1 - go<S>(q, 0)] + a1[1 - go<S>(q, 1)]
+                                          : a0[1] + a1[1]);
+    T x0, x1;
+    if (ez == 0) {
+      const T2 v = *(const T2*)(row + z0);
+      x0 = v.x;
+      x1 = v.y;
+    } else if (ez > 0) {  // cells z0, z0+1 pull from z0-1, z0
+      if (!lo_edge) {
+        const T2 v = *(const T2*)(row + z0 - 1);
+        x0 = v.x;
+        x1 = v.y;
+      } else {
+        x0 = row[p.n2 - 1];
+        x1 = row[z0];
+      }
+    } else {  // cells z0, z0+1 pull from z0+1, z0+2
+      if (!hi_edge) {
+        const T2 v = *(const T2*)(row + z0 + 1);
+        x0 = v.x;
+        x1 = v.y;
+      } else {
+        x0 = row[z0 + 1];
+        x1 = row[0];
+      }
+    }
+    f0[q] = ST::dec(x0, sh);
+    f1[q] = ST::dec(x1, sh);
+  }
+  const CollParams<C> cp = coll_params<C>(p);
+  collide<S, C, COLL>(f0, cp);
+  collide<S, C, COLL>(f1, cp);
+  T* fout = (T*)p.fout + a0[1] + a1[1] + z0;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C sh = shift_of<ST>(S::w[q], p);
+    T2 v;
+    v.x = ST::enc(f0[q], sh);
+    v.y = ST::enc(f1[q], sh);
+    T2* dst = (T2*)(fout + q * p.sq);
+    if (NT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 using KernelFn = void (*)(const KParams);
+
+template <class S, class ST, int COLL>
+KernelFn pick_pair_st(bool pull, bool nt) {
+  if (pull) return nt ? pair_kernel<S, ST, COLL, true, true> : pair_kernel<S, ST, COLL, true, false>;
+  return nt ? pair_kernel<S, ST, COLL, false, true> : pair_kernel<S, ST, COLL, false, false>;
+}
+
+template <class S, class ST>
+KernelFn pick_pair_coll(int coll, bool pull, bool nt) {
+  switch (coll) {
+    case LBM_COLL_NONE: return pick_pair_st<S, ST, LBM_COLL_NONE>(pull, nt);
+    case LBM_COLL_BGK: return pick_pair_st<S, ST, LBM_COLL_BGK>(pull, nt);
+    case LBM_COLL_TRT: return pick_pair_st<S, ST, LBM_COLL_TRT>(pull, nt);
+  }
+  return nullptr;
+}
+
+// pair kernels exist for the unmasked pull / no-streaming paths of BGK, TRT
+// and no collision
+template <class S>
+KernelFn pick_pair_kernel(int storage, int coll, int mode, bool nt) {
+  if (mode != LBM_PRE_STREAMING && mode != LBM_NO_STREAMING) return nullptr;
+  const bool pull = mode == LBM_PRE_STREAMING;
+  switch (storage) {
+    case LBM_F32: return pick_pair_coll<S, StF32>(coll, pull, nt);
+    case LBM_F64: return pick_pair_coll<S, StF64>(coll, pull, nt);
+    case LBM_F16_SHIFTED: return pick_pair_coll<S, StF16>(coll, pull, nt);
+    case LBM_BF16_SHIFTED: return pick_pair_coll<S, StBF16>(coll, pull, nt);
+  }
+  return nullptr;
+}
 
 template <class S, class ST, int COLL, bool MASK>
 KernelFn pick_mode(int mode) {

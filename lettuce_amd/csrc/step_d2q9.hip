@@ -5,4 +5,7 @@ namespace lbm {
 KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode) {
   return pick_kernel<D2Q9>(storage, coll, mask, mode);
 }
+KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt) {
+  return pick_pair_kernel<D2Q9>(storage, coll, mode, nt);
+}
 }  // namespace lbm

@@ -1,0 +1,170 @@
+"""Accuracy studies on the synthetic flows, each against an FP64 run of the
+same case on the GPU.  Prints one JSON object per study.
+
+    python -m lettuce_amd.studies tgv2d        # D2Q9 TGV 128^2 vs analytic decay
+    python -m lettuce_amd.studies tgv3d        # D3Q19 TGV 256^3, FP32 / FP16 / BF16 storage
+    python -m lettuce_amd.studies cylinder     # D2Q9 channel + cylinder, KBC vs learned
+    python -m lettuce_amd.studies all
+
+Options: --steps N overrides a study's step count; --res N its resolution.
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+import torch
+
+import lettuce_amd as lt
+
+
+def _ke_series(sim, flow, steps, every):
+    """(steps, KE, enstrophy) every `every` steps, FP64 reductions."""
+    ke = lt.IncompressibleKineticEnergy(flow)
+    en = lt.Enstrophy(flow)
+    out = [(0, float(ke()), float(en()))]
+    t = 0
+    while t < steps:
+        k = min(every, steps - t)
+        sim(k)
+        t += k
+        out.append((t, float(ke()), float(en())))
+    return np.array(out)
+
+
+def tgv2d(steps=1000, res=128):
+    """D2Q9 BGK Taylor-Green vortex, Re 100, u0 = 0.05 (Ma = 0.05 sqrt 3),
+    FP32; kinetic energy against the analytic exp(-4 nu t) (lettuce's 2D
+    TGV velocity decays as exp(-2 nu t), taylorgreen.py:72-81) and against an
+    FP64 run."""
+    res_ = {}
+    for name, dtype in (("f32", torch.float32), ("f64", torch.float64)):
+        flow = lt.TaylorGreenVortex(lt.Context("cuda:0", dtype=dtype), res, 100,
+                                    0.05 * math.sqrt(3), lt.D2Q9(), initialize_fneq=False)
+        sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                            lt.StreamingStrategy.PRE_STREAMING)
+        res_[name] = _ke_series(sim, flow, steps, 100)
+        units = flow.units
+    t_pu = units.convert_time_to_pu(res_["f32"][:, 0])
+    analytic = res_["f32"][0, 1] * np.exp(-4 * units.viscosity_pu * t_pu)
+    return {"study": "tgv2d_d2q9_bgk", "grid": [res, res], "steps": steps,
+            "ke_ratio_f32": float(res_["f32"][-1, 1] / res_["f32"][0, 1]),
+            "ke_ratio_analytic": float(analytic[-1] / analytic[0]),
+            "ke_rel_err_vs_analytic_max": float(np.max(np.abs(res_["f32"][:, 1] / analytic - 1))),
+            "ke_rel_err_f32_vs_f64_max": float(np.max(np.abs(res_["f32"][:, 1] / res_["f64"][:, 1] - 1)))}
+
+
+def tgv3d(steps=2000, res=256):
+    """D3Q19 BGK TGV, Re 1600, Ma 0.1: FP32 storage and shifted FP16/BF16
+    storage (FP32 arithmetic) against FP64; KE and enstrophy every 50 steps."""
+    series, timing = {}, {}
+    for name, dtype, storage in (("f64", torch.float64, "f64"), ("f32", torch.float32, "f32"),
+                                 ("f16_shifted", torch.float32, "f16_shifted"),
+                                 ("bf16_shifted", torch.float32, "bf16_shifted")):
+        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+        flow = lt.TaylorGreenVortex(ctx, res, 1600, 0.1, lt.D3Q19(), initialize_fneq=False)
+        sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                            lt.StreamingStrategy.PRE_STREAMING)
+        t0 = time.perf_counter()
+        series[name] = _ke_series(sim, flow, steps, 50)
+        torch.cuda.synchronize()
+        timing[name] = time.perf_counter() - t0
+        del sim, flow
+        torch.cuda.empty_cache()
+    ref = series["f64"]
+    out = {"study": "tgv3d_d3q19_bgk", "grid": [res] * 3, "steps": steps, "every": 50}
+    for name in ("f32", "f16_shifted", "bf16_shifted"):
+        s = series[name]
+        out[name] = {"ke_rel_err_max": float(np.max(np.abs(s[:, 1] / ref[:, 1] - 1))),
+                     "enstrophy_rel_err_max": float(np.max(np.abs(s[:, 2] / ref[:, 2] - 1))),
+                     "ke_final": float(s[-1, 1]), "enstrophy_final": float(s[-1, 2]),
+                     "wall_s_incl_reporting": round(timing[name], 3)}
+    out["f64"] = {"ke_final": float(ref[-1, 1]), "enstrophy_final": float(ref[-1, 2])}
+    return out
+
+
+def _strouhal(lift, dt_pu, char_length_pu=1.0, char_velocity_pu=1.0):
+    """Dominant lift frequency by FFT of the mean-free second half."""
+    x = np.asarray(lift[len(lift) // 2:], dtype=np.float64)
+    x = x - x.mean()
+    if len(x) < 8 or not np.any(x):
+        return float("nan")
+    spec = np.abs(np.fft.rfft(x * np.hanning(len(x))))
+    freqs = np.fft.rfftfreq(len(x), d=dt_pu)
+    k = int(np.argmax(spec[1:]) + 1)
+    return float(freqs[k] * char_length_pu / char_velocity_pu)
+
+
+def cylinder(steps=40000, nx=2048, ny=512, diameter=64, every=50, storage="f32",
+             collision="kbc", seed=0, dtype=torch.float32):
+    """D2Q9 channel with a cylinder of `diameter` cells centred at
+    (nx/4, ny/2 + 1): halfway bounce-back on cylinder and walls, equilibrium
+    inlet u = (0.05, 0), equilibrium outlet, Re 200 on the diameter.
+    Drag and lift coefficients by momentum exchange every `every` steps,
+    Strouhal number from the lift signal."""
+    ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+    u_lu = 0.05
+    ma = u_lu * math.sqrt(3)
+    flow = lt.Obstacle(ctx, [nx, ny], 200, ma, domain_length_x=nx / diameter,
+                       stencil=lt.D2Q9(), outlet="equilibrium", solid="halfway", walls=True)
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+    cx, cy = nx / 4.0, ny / 2.0 + 1.0
+    flow.mask = (xx - cx) ** 2 + (yy - cy) ** 2 < (diameter / 2.0) ** 2
+    flow.initialize()
+    coll = lt.KBCCollision() if collision == "kbc" else lt.LearnedCollision(
+        lt.LearnedParameters.seeded(seed, out_scale=0.1))
+    sim = lt.Simulation(flow, coll, [], lt.StreamingStrategy.PRE_STREAMING)
+    hw = sim.post_streaming_boundaries[0]
+    # the force on the cylinder only: links into the cylinder's cells
+    cyl = torch.as_tensor(flow.mask, dtype=torch.bool)
+    from lettuce_amd._classes import halfway_links
+    links = halfway_links(cyl, flow.stencil.e, flow.stencil.opposite)
+    norm = 0.5 * 1.0 * u_lu ** 2 * diameter
+    cd, cl = [], []
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        sim(k)
+        done += k
+        F = lt._ops.link_force_f64(flow.stencil, flow._state, links, flow.rho0_shift)
+        cd.append(float(F[0]) / norm)
+        cl.append(float(F[1]) / norm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dt_pu = flow.units.convert_time_to_pu(every)
+    half = len(cd) // 2
+    return {"study": "cylinder_d2q9", "collision": collision, "storage": storage,
+            "grid": [nx, ny], "diameter": diameter, "steps": steps,
+            "tau": flow.units.relaxation_parameter_lu,
+            "cd_mean": float(np.mean(cd[half:])) if cd else None,
+            "cl_amplitude": float(0.5 * (np.max(cl[half:]) - np.min(cl[half:]))) if cl else None,
+            "strouhal": _strouhal(cl, dt_pu),
+            "finite": bool(np.all(np.isfinite(cd + cl))),
+            "wall_s_incl_forces": round(wall, 3),
+            "mlups_incl_forces": round(steps * nx * ny / 1e6 / wall, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["tgv2d", "tgv3d", "cylinder", "all"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--res", type=int, default=None)
+    args = ap.parse_args()
+    kw = {}
+    if args.steps:
+        kw["steps"] = args.steps
+    if args.study in ("tgv2d", "all"):
+        print(json.dumps(tgv2d(**kw, **({"res": args.res} if args.res else {}))), flush=True)
+    if args.study in ("tgv3d", "all"):
+        print(json.dumps(tgv3d(**kw, **({"res": args.res} if args.res else {}))), flush=True)
+    if args.study in ("cylinder", "all"):
+        for coll, storage in (("kbc", "f32"), ("learned", "f32"), ("learned", "f16_shifted")):
+            print(json.dumps(cylinder(collision=coll, storage=storage, **kw)), flush=True)
+        ref = cylinder(collision="kbc", storage="f64", dtype=torch.float64, **kw)
+        print(json.dumps(ref), flush=True)
+
+
+if __name__ == "__main__":
+    main()
