@@ -32,19 +32,38 @@ WORKLOADS = {
     "tgv3d_d3q19_bgk_256_f16s": ("D3Q19", "bgk", "f16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_bgk_256_bf16s": ("D3Q19", "bgk", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q27_kbc_256_f32": ("D3Q27", "kbc", "f32", 256, 1600, 0.1),
+    "tgv3d_d3q19_none_256_f32": ("D3Q19", "none", "f32", 256, 1600, 0.1),
     "tgv2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.1),
+    # channel + cylinder (D = 64, halfway bounce-back, equilibrium in/outlet)
+    "cyl2d_d2q9_kbc_2048x512_f32": ("D2Q9", "kbc", "f32", [2048, 512], 200, 0.05 * 3 ** 0.5),
+    "cyl2d_d2q9_learned_2048x512_f32": ("D2Q9", "learned", "f32", [2048, 512], 200,
+                                        0.05 * 3 ** 0.5),
+    "cyl2d_d2q9_learned_2048x512_f16s": ("D2Q9", "learned", "f16_shifted", [2048, 512], 200,
+                                         0.05 * 3 ** 0.5),
+    "cyl2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.05 * 3 ** 0.5),
 }
 BYTES = {"f32": 4, "f64": 8, "f16_shifted": 2, "bf16_shifted": 2}
 
 
-def build_sim(lt, name):
+def build_sim(lt, name, layout="soa"):
     stencil, coll, storage, res, re, ma = WORKLOADS[name]
     S = {"D2Q9": lt.D2Q9, "D3Q19": lt.D3Q19, "D3Q27": lt.D3Q27}[stencil]()
-    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage)
-    flow = lt.TaylorGreenVortex(ctx, res, re, ma, S, initialize_fneq=False)
+    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage, layout=layout)
+    if name.startswith("cyl2d"):
+        nx, ny = res
+        flow = lt.Obstacle(ctx, [nx, ny], re, ma, domain_length_x=nx / 64, stencil=S,
+                           outlet="equilibrium", solid="halfway", walls=True)
+        yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+        flow.mask = (xx - nx / 4) ** 2 + (yy - ny / 2 - 1) ** 2 < 32.0 ** 2
+        flow.initialize()
+    else:
+        flow = lt.TaylorGreenVortex(ctx, res, re, ma, S, initialize_fneq=False)
     tau = flow.units.relaxation_parameter_lu
-    collision = {"bgk": lt.BGKCollision(tau), "kbc": lt.KBCCollision(),
-                 "trt": lt.TRTCollision(tau, 1.0)}[coll]
+    collision = {"bgk": lambda: lt.BGKCollision(tau), "kbc": lambda: lt.KBCCollision(),
+                 "none": lambda: lt.NoCollision(),
+                 "trt": lambda: lt.TRTCollision(tau, 1.0),
+                 "learned": lambda: lt.LearnedCollision(
+                     lt.LearnedParameters.seeded(0, out_scale=0.1))}[coll]()
     sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
     return sim, S
 
@@ -75,10 +94,19 @@ def cpu_baseline(name, budget_s=15.0):
     sample = [512, 128] if S.d == 2 else 96
     threads = os.cpu_count() or 1
     torch.set_num_threads(threads)
-    flow = O.TaylorGreenVortex(O.Context("cpu", torch.float32), sample, re, ma, S)
+    if name.startswith("cyl2d"):
+        yy, xx = np.meshgrid(np.arange(128), np.arange(512))
+        mask = (xx - 128) ** 2 + (yy - 65) ** 2 < 8.0 ** 2
+        flow = O.ObstacleChannel(O.Context("cpu", torch.float32), sample, re, ma,
+                                 domain_length_x=512 / 16, mask=mask, stencil=S)
+    else:
+        flow = O.TaylorGreenVortex(O.Context("cpu", torch.float32), sample, re, ma, S)
     tau = flow.units.relaxation_parameter_lu
-    c = {"bgk": O.BGKCollision(tau), "kbc": O.KBCCollision(),
-         "trt": O.TRTCollision(tau, 1.0)}[coll]
+    c = {"bgk": lambda: O.BGKCollision(tau), "kbc": lambda: O.KBCCollision(),
+         "none": lambda: O.NoCollision(),
+         "trt": lambda: O.TRTCollision(tau, 1.0),
+         "learned": lambda: O.LearnedCollision(O.LearnedParameters.seeded(0, out_scale=0.1))
+         }[coll]()
     sim = O.Simulation(flow, c, [], O.StreamingStrategy.PRE_STREAMING)
     sim(1)  # warm-up
     cells = int(np.prod(flow.resolution))
@@ -91,7 +119,8 @@ def cpu_baseline(name, budget_s=15.0):
             break
     return {"value": round(steps * cells / 1e6 / el, 3), "unit": "MLUPS",
             "cores": threads, "kind": "port",
-            "sample": f"{stencil} {coll.upper()} TGV {'x'.join(map(str, flow.resolution))} "
+            "sample": f"{stencil} {coll.upper()} {type(flow).__name__} "
+                      f"{'x'.join(map(str, flow.resolution))} "
                       f"FP32, {steps} steps after 1 warm-up, torch CPU "
                       f"({torch.get_num_threads()} threads)"}
 
@@ -152,6 +181,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy", action="store_true",
                     help="only time a device copy of one population array")
+    ap.add_argument("--layout", default="soa", choices=["soa", "rows"],
+                    help="population memory order (see lettuce_amd.Context)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -171,12 +202,13 @@ def main():
         return
 
     import lettuce_amd as lt
-    sim, S = build_sim(lt, args.workload)
+    sim, S = build_sim(lt, args.workload, args.layout)
     stencil, coll, storage, res, re, ma = WORKLOADS[args.workload]
     cells = int(np.prod(sim.flow.resolution))
     gpu_s, wall_s = time_gpu(sim, args.steps, args.warmup)
     mlups = args.steps * cells / 1e6 / gpu_s
-    bytes_per_cell = 2 * S.q * BYTES[storage]
+    # populations read + written once; the uint8 cell class read once
+    bytes_per_cell = 2 * S.q * BYTES[storage] + (1 if sim.cell_classes is not None else 0)
     launch_s = gpu_s / args.steps
     achieved = bytes_per_cell * cells / launch_s / 1e9
     traffic = load_traffic(args.workload)
@@ -196,7 +228,8 @@ def main():
         "config": {"workload": args.workload, "lattice": stencil,
                    "collision": coll.upper(), "storage": storage,
                    "grid": sim.flow.resolution, "cells": cells,
-                   "streaming": "pre (pull), fused", "Re": re, "Ma": ma},
+                   "streaming": "pre (pull), fused", "Re": re, "Ma": ma,
+                   "layout": args.layout},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),

@@ -9,6 +9,10 @@ constructor arguments, defaults and tensor factories.  Differences:
 * ``storage`` selects how populations are held in HBM: ``"f32"``/``"f64"``
   (the dtype itself) or ``"f16_shifted"``/``"bf16_shifted"`` (f_q - w_q rho0
   in 16 bits, FP32 arithmetic).  lettuce has no such option.
+* ``layout`` selects the memory order behind the [Q, *res] view of the
+  populations: ``"soa"`` (lettuce's contiguous [Q, *res]) or ``"rows"``
+  (each row of the last axis stored for all Q populations together,
+  [*res[:-1], Q, res[-1]]; periodic flows without boundaries only).
 """
 from typing import List, Optional, Union
 
@@ -26,7 +30,7 @@ class Context:
     def __init__(self, device: Optional[Union[torch.device, str]] = None,
                  dtype: Optional[torch.dtype] = None,
                  use_native: Optional[bool] = None,
-                 storage: Optional[str] = None):
+                 storage: Optional[str] = None, layout: str = "soa"):
         cuda = torch.cuda.is_available()
         if device is None:
             if use_native is None:
@@ -66,10 +70,26 @@ class Context:
         if dtype == torch.float32 and storage == "f64":
             raise ValueError("storage='f64' needs dtype=torch.float64")
 
+        if layout not in ("soa", "rows"):
+            raise ValueError(f"unknown layout {layout!r}; expected 'soa' or 'rows'")
+
         self.device = device
         self.dtype = dtype
         self.use_native = use_native
         self.storage = storage
+        self.layout = layout
+
+    def population_tensor(self, q: int, resolution) -> torch.Tensor:
+        """Uninitialised populations [q, *resolution] in the storage dtype,
+        laid out as `layout` says."""
+        res = list(resolution)
+        if self.layout == "soa":
+            return torch.empty([q] + res, device=self.device,
+                               dtype=self.storage_dtype)
+        raw = torch.empty(res[:-1] + [q, res[-1]], device=self.device,
+                          dtype=self.storage_dtype)
+        d = len(res)
+        return raw.permute([d - 1] + list(range(d - 1)) + [d])
 
     @property
     def storage_dtype(self) -> torch.dtype:

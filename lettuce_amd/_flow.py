@@ -73,8 +73,7 @@ class Flow(ABC):
         self.torch_stencil = TorchStencil(stencil, context)
         self.equilibrium = equilibrium
         self.i = 0
-        self._state = context.empty_tensor([stencil.q, *self.resolution],
-                                           dtype=context.storage_dtype)
+        self._state = context.population_tensor(stencil.q, self.resolution)
         self._state_next = None
         self.initialize()
 
@@ -140,15 +139,21 @@ class Flow(ABC):
                              f"got {tuple(value.shape)}")
         if self.context.shifted:
             _ops.encode(self.stencil, value, self._state, self.rho0_shift)
-        else:
+        elif self.context.layout == "soa":
             self._state = value.contiguous()
+        else:
+            state = self.context.population_tensor(self.stencil.q, self.resolution)
+            state.copy_(value)
+            self._state = state
 
     @property
     def f_next(self) -> torch.Tensor:
         """The second (B) buffer, created on first use
         (lettuce/_flow.py:145-151); in the storage format."""
-        if self._state_next is None or self._state_next.shape != self._state.shape:
-            self._state_next = torch.empty_like(self._state)
+        if (self._state_next is None or self._state_next.shape != self._state.shape
+                or self._state_next.stride() != self._state.stride()):
+            self._state_next = self.context.population_tensor(self.stencil.q,
+                                                              self.resolution)
         return self._state_next
 
     @f_next.setter

@@ -23,7 +23,8 @@ KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
 }  // namespace lbm
 
 // Launch tuning read from the environment (for measurements):
-//  LETTUCE_AMD_CELLS_PER_THREAD=1|2  pair kernel where available (default 2)
+//  LETTUCE_AMD_CELLS_PER_THREAD=1|2  pair kernel where available (default 1:
+//                                    measured slower on MI355X, 5.30 vs 5.68 TB/s)
 //  LETTUCE_AMD_NT_STORE=0|1          non-temporal stores in the pair kernel (default 1)
 static int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
@@ -82,15 +83,35 @@ static int check_layout(int stencil, int storage, const int64_t n[3], const int6
   if (stencil_d(stencil) == 2 && n[0] != 1)
     return fail(LBM_E_ARG, "2D lattices need n[0] == 1, got %lld", (long long)n[0]);
   if (stride[3] != 1) return fail(LBM_E_ARG, "the last axis must be contiguous (stride 1)");
-  if (stride[2] < n[2] || (n[0] > 1 && stride[1] < stride[2] * n[1]))
-    return fail(LBM_E_ARG, "overlapping spatial strides");
-  const int64_t span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
-  if (stride[0] < span) return fail(LBM_E_ARG, "population stride %lld smaller than one population (%lld)",
-                                    (long long)stride[0], (long long)span);
-  // all cell offsets are 32-bit inside the kernels
-  if (span > (int64_t)0x7fffffff) return fail(LBM_E_ARG, "grid too large for 32-bit cell offsets");
-  if ((int64_t)Q * stride[0] > ((int64_t)1 << 40)) return fail(LBM_E_ARG, "population array too large");
+  // the strides must address every (q, i0, i1, i2) at a distinct element:
+  // sorted by stride, each axis must step over the whole extent of the
+  // axes below it
+  int64_t ext[4] = {Q, n[0], n[1], n[2]}, str[4] = {stride[0], stride[1], stride[2], 1};
+  int ord[4] = {3, 2, 1, 0};
+  for (int i = 1; i < 4; ++i)
+    for (int j = i; j > 0 && str[ord[j]] < str[ord[j - 1]]; --j) {
+      const int t = ord[j];
+      ord[j] = ord[j - 1];
+      ord[j - 1] = t;
+    }
+  int64_t span = 1;
+  for (int k = 0; k < 4; ++k) {
+    const int a = ord[k];
+    if (ext[a] == 1) continue;
+    if (str[a] < span) return fail(LBM_E_ARG, "overlapping population strides");
+    span = str[a] * ext[a];
+  }
+  // cell offsets are 32-bit inside the kernels; populations use 64-bit bases
+  const int64_t cell_span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
+  if (cell_span > (int64_t)0x7fffffff) return fail(LBM_E_ARG, "grid too large for 32-bit cell offsets");
+  if (span > ((int64_t)1 << 40)) return fail(LBM_E_ARG, "population array too large");
   return LBM_OK;
+}
+
+// masked kernels index the class map by the storage offset of a cell, which
+// is its linear index only for the contiguous [Q][n0][n1][n2] layout
+static bool contiguous_cells(const int64_t n[3], const int64_t stride[4]) {
+  return stride[2] == n[2] && (n[0] == 1 || stride[1] == n[1] * n[2]);
 }
 
 static int block_size_for(int64_t n2) {
@@ -132,6 +153,8 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   const bool mask = a->cell_class != nullptr;
   if (mask && (!a->classes || a->n_classes < 1 || a->n_classes > 256))
     return fail(LBM_E_ARG, "cell_class given without a class table of 1..256 entries");
+  if (mask && !contiguous_cells(a->n, a->stride))
+    return fail(LBM_E_UNSUPPORTED, "boundaries need each population stored as one contiguous block");
   if (a->fallback_count_ptr_reserved != 0) return fail(LBM_E_ARG, "reserved field must be 0");
   if (a->collision == LBM_COLL_BGK && !(a->tau > 0)) return fail(LBM_E_ARG, "BGK needs tau > 0");
   if (a->collision == LBM_COLL_TRT && !(a->tau > 0 && a->tau_minus > 0))
@@ -174,9 +197,10 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.s1 = (int)a->stride[2];
   p.sq = a->stride[0];
   p.ncells = a->n[0] * a->n[1] * a->n[2];
-  // two cells per thread where a pair kernel exists and rows have even length
+  // two cells per thread where requested, a pair kernel exists and rows
+  // have even length
   int64_t row_threads = a->n[2];
-  if (fn && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 2) == 2) {
+  if (fn && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 1) == 2) {
     const bool nt = env_int("LETTUCE_AMD_NT_STORE", 1) != 0;
     KernelFn pf = nullptr;
     switch (a->stencil) {
