@@ -181,8 +181,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy", action="store_true",
                     help="only time a device copy of one population array")
-    ap.add_argument("--layout", default="soa", choices=["soa", "rows"],
-                    help="population memory order (see lettuce_amd.Context)")
+    ap.add_argument("--layout", default=None, choices=["soa", "rows"],
+                    help="population memory order (see lettuce_amd.Context); default "
+                         "'rows' for periodic flows, 'soa' for flows with boundaries")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -202,6 +203,8 @@ def main():
         return
 
     import lettuce_amd as lt
+    if args.layout is None:
+        args.layout = "soa" if args.workload.startswith("cyl2d") else "rows"
     sim, S = build_sim(lt, args.workload, args.layout)
     stencil, coll, storage, res, re, ma = WORKLOADS[args.workload]
     cells = int(np.prod(sim.flow.resolution))

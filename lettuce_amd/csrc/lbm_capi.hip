@@ -20,12 +20,19 @@ int launch_learned(const KParams& p, const lbm_learned* l, int storage, bool mas
 KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q19(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
+KernelFn pick_row_kernel_d2q9(int storage, int coll);
+KernelFn pick_row_kernel_d3q19(int storage, int coll);
+KernelFn pick_row_kernel_d3q27(int storage, int coll);
 }  // namespace lbm
 
 // Launch tuning read from the environment (for measurements):
 //  LETTUCE_AMD_CELLS_PER_THREAD=1|2  pair kernel where available (default 1:
 //                                    measured slower on MI355X, 5.30 vs 5.68 TB/s)
 //  LETTUCE_AMD_NT_STORE=0|1          non-temporal stores in the pair kernel (default 1)
+//  LETTUCE_AMD_ROW_KERNEL=0|1        one block per row, aligned loads and lane
+//                                    shifts, where rows are 64..1024 long (default 0:
+//                                    measured equal for FP32 BGK, slower for FP16
+//                                    storage and KBC)
 static int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
   return v ? atoi(v) : dflt;
@@ -123,9 +130,9 @@ static int block_size_for(int64_t n2) {
   return 256;
 }
 
-// Block order for a grid of nb blocks: the XCD chunk order when the rows
-// split evenly over the 8 XCDs, else the XCD slab order, else storage order.
-// LETTUCE_AMD_BLOCK_ORDER=linear|slab|chunk overrides it where valid.
+// Block order for a grid of nb blocks: storage order unless
+// LETTUCE_AMD_BLOCK_ORDER=slab|chunk asks for an XCD-aware order (and the
+// grid splits evenly over the 8 XCDs).
 static int block_order(const KParams& p, int64_t nb) {
   const bool slab_ok = nb % 8 == 0 && nb >= 64;
   const bool chunk_ok = slab_ok && p.n1 % 8 == 0;
@@ -213,7 +220,22 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
       row_threads = a->n[2] / 2;
     }
   }
-  const int bs = block_size_for(row_threads);
+  // one block per row with aligned loads and lane shifts along i2
+  bool row_kernel = false;
+  if (fn && !mask && a->streaming == LBM_PRE_STREAMING && a->n[2] % 64 == 0 && a->n[2] <= 1024 &&
+      env_int("LETTUCE_AMD_ROW_KERNEL", 0) != 0 && row_threads == a->n[2]) {
+    KernelFn rf = nullptr;
+    switch (a->stencil) {
+      case LBM_D2Q9: rf = pick_row_kernel_d2q9(a->storage, a->collision); break;
+      case LBM_D3Q19: rf = pick_row_kernel_d3q19(a->storage, a->collision); break;
+      case LBM_D3Q27: rf = pick_row_kernel_d3q27(a->storage, a->collision); break;
+    }
+    if (rf) {
+      fn = rf;
+      row_kernel = true;
+    }
+  }
+  const int bs = row_kernel ? (int)a->n[2] : block_size_for(row_threads);
   p.nb2 = (int)((row_threads + bs - 1) / bs);
   const int64_t nb = (int64_t)p.nb2 * a->n[1] * a->n[0];
   if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");

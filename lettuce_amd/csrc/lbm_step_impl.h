@@ -352,10 +352,109 @@ __global__ void __launch_bounds__(256) pair_kernel(const KParams p) {
   }
 }
 
+// Shift a value across the lanes of the wavefront with DPP: DIR = +1 gives
+// lane i the value of lane i - 1 (wave_shr:1), DIR = -1 the value of lane
+// i + 1 (wave_shl:1).  The lane at the wave's edge keeps its own value.
+template <int DIR>
+__device__ __forceinline__ int wave_shift_i32(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, DIR > 0 ? 0x138 : 0x130, 0xf, 0xf, false);
+}
+template <int DIR, class T>
+__device__ __forceinline__ T wave_shift(T v) {
+  if constexpr (sizeof(T) == 8) {
+    long long b = __builtin_bit_cast(long long, v);
+    const int lo = wave_shift_i32<DIR>((int)(b & 0xffffffffll));
+    const int hi = wave_shift_i32<DIR>((int)(b >> 32));
+    return __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned int)lo);
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, wave_shift_i32<DIR>(__builtin_bit_cast(int, v)));
+  } else {
+    const unsigned short s = __builtin_bit_cast(unsigned short, v);
+    return __builtin_bit_cast(T, (unsigned short)wave_shift_i32<DIR>((int)s));
+  }
+}
+
+// Unmasked pull step with one block per row along i2 (blockDim.x == n2, a
+// multiple of 64): every population is loaded aligned at the thread's own
+// i2; the +-1 shift along i2 is a lane shift inside the wavefront, and the
+// element that crosses a wavefront (or the periodic row end) goes through
+// two LDS slots per wave and population.  Same arithmetic as step_kernel.
+template <class S, class ST, int COLL>
+__global__ void __launch_bounds__(1024) row_kernel(const KParams p) {
+  using C = typename ST::C;
+  using T = typename ST::T;
+  __shared__ T edge_last[S::Q][16];   // last element of each wave
+  __shared__ T edge_first[S::Q][16];  // first element of each wave
+  const int row = blockIdx.x;
+  const int i1 = row % p.n1, i0 = row / p.n1;
+  const int z = threadIdx.x, lane = z & 63, wid = z >> 6;
+  const int nw = blockDim.x >> 6;
+  int a0[3], a1[3];
+  axis_offsets(i0, p.n0, p.s0, a0);
+  axis_offsets(i1, p.n1, p.s1, a1);
+  const T* fin = (const T*)p.fin;
+  T raw[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const T v = fin[q * p.sq + a0[1 - go<S>(q, 0)] + a1[1 - go<S>(q, 1)] + z];
+    const int ez = go<S>(q, 2);
+    if (ez == 0) {
+      raw[q] = v;
+    } else if (ez > 0) {  // needs z - 1
+      raw[q] = wave_shift<+1>(v);
+      if (lane == 63) edge_last[q][wid] = v;
+    } else {  // needs z + 1
+      raw[q] = wave_shift<-1>(v);
+      if (lane == 0) edge_first[q][wid] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const int ez = go<S>(q, 2);
+    if (ez > 0 && lane == 0) raw[q] = edge_last[q][wid == 0 ? nw - 1 : wid - 1];
+    if (ez < 0 && lane == 63) raw[q] = edge_first[q][wid == nw - 1 ? 0 : wid + 1];
+  }
+  C f[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+  collide<S, C, COLL>(f, coll_params<C>(p));
+  T* fout = (T*)p.fout;
+  const int cell = a0[1] + a1[1] + z;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) ST::st(fout + q * p.sq, cell, f[q], shift_of<ST>(S::w[q], p));
+}
+
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 using KernelFn = void (*)(const KParams);
+
+template <class S, class ST>
+KernelFn pick_row_coll(int coll) {
+  switch (coll) {
+    case LBM_COLL_NONE: return row_kernel<S, ST, LBM_COLL_NONE>;
+    case LBM_COLL_BGK: return row_kernel<S, ST, LBM_COLL_BGK>;
+    case LBM_COLL_TRT: return row_kernel<S, ST, LBM_COLL_TRT>;
+    case LBM_COLL_KBC:
+      if constexpr (S::ID == LBM_D3Q19) return nullptr;
+      else return row_kernel<S, ST, LBM_COLL_KBC>;
+  }
+  return nullptr;
+}
+
+// row kernels exist for the unmasked pull path of every collision but the
+// learned one
+template <class S>
+KernelFn pick_row_kernel(int storage, int coll) {
+  switch (storage) {
+    case LBM_F32: return pick_row_coll<S, StF32>(coll);
+    case LBM_F64: return pick_row_coll<S, StF64>(coll);
+    case LBM_F16_SHIFTED: return pick_row_coll<S, StF16>(coll);
+    case LBM_BF16_SHIFTED: return pick_row_coll<S, StBF16>(coll);
+  }
+  return nullptr;
+}
 
 template <class S, class ST, int COLL>
 KernelFn pick_pair_st(bool pull, bool nt) {

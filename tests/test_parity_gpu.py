@@ -402,14 +402,19 @@ def test_run_batches_equal_single_steps():
 
 
 @pytest.mark.parametrize("name,res", [("D3Q19", [8, 6, 10]), ("D2Q9", [12, 2]),
-                                      ("D3Q27", [4, 4, 6]), ("D2Q9", [5, 7])])
+                                      ("D3Q27", [4, 4, 6]), ("D2Q9", [5, 7]),
+                                      ("D3Q19", [4, 6, 128]), ("D2Q9", [12, 64]),
+                                      ("D3Q27", [3, 4, 192])])
 @pytest.mark.parametrize("storage,dtype", [("f32", F32), ("f64", F64),
                                            ("bf16_shifted", F32)])
-def test_pair_kernel_matches_single_cell_kernel(name, res, storage, dtype, monkeypatch):
-    """Two-cells-per-thread and one-cell-per-thread kernels give identical
-    populations (same per-cell arithmetic; only the memory access differs),
-    including the periodic wrap of the pair at the row ends and odd rows
-    (which always take the single-cell kernel)."""
+@pytest.mark.parametrize("variant", ["LETTUCE_AMD_CELLS_PER_THREAD", "LETTUCE_AMD_ROW_KERNEL"])
+def test_kernel_variants_match_single_cell_kernel(name, res, storage, dtype, variant,
+                                                  monkeypatch):
+    """The two-cells-per-thread kernel and the row kernel (aligned loads and
+    lane shifts, rows of 64..1024) give the same populations as the
+    one-cell-per-thread kernel bit for bit (same per-cell arithmetic; only
+    the memory access differs), including the periodic wrap at the row ends
+    and rows the variants do not apply to."""
     s, os_ = _stencils()[name]
     g = torch.Generator().manual_seed(9)
     from tests.test_oracle import _Grid
@@ -418,12 +423,17 @@ def test_pair_kernel_matches_single_cell_kernel(name, res, storage, dtype, monke
     f0 = oflow.equilibrium(oflow, rho=torch.ones([1] + res, dtype=F64), u=u)
     f0 = (f0 * (1 + 0.02 * torch.rand(f0.shape, generator=g, dtype=F64))).numpy()
     outs = []
-    for cpt in ("1", "2"):
-        monkeypatch.setenv("LETTUCE_AMD_CELLS_PER_THREAD", cpt)
+    monkeypatch.setenv("LETTUCE_AMD_CELLS_PER_THREAD", "1")
+    monkeypatch.setenv("LETTUCE_AMD_ROW_KERNEL", "0")
+    for on in ("0", "2" if variant == "LETTUCE_AMD_CELLS_PER_THREAD" else "1"):
+        if on != "0":
+            monkeypatch.setenv(variant, on)
         for strategy in ("PRE_STREAMING", "NO_STREAMING"):
             flow = ArrayFlow(ctx(dtype, storage), f0, s)
-            outs.append(native_run(flow, lt.BGKCollision(0.7), 5,
-                                   getattr(lt.StreamingStrategy, strategy)))
+            coll = lt.TRTCollision(0.7, 0.9) if name == "D3Q19" else lt.KBCCollision()
+            if variant == "LETTUCE_AMD_CELLS_PER_THREAD":
+                coll = lt.BGKCollision(0.7)   # pair kernels: BGK, TRT, none
+            outs.append(native_run(flow, coll, 5, getattr(lt.StreamingStrategy, strategy)))
     assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
 
 
