@@ -11,6 +11,7 @@ Options: --steps N overrides a study's step count; --res N its resolution.
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -84,16 +85,18 @@ def tgv3d(steps=2000, res=256):
     return out
 
 
-def _strouhal(lift, dt_pu, char_length_pu=1.0, char_velocity_pu=1.0):
-    """Dominant lift frequency by FFT of the mean-free second half."""
+def _strouhal(lift, every, diameter, u_lu):
+    """St = f D / U from the lift signal (samples `every` steps apart) over
+    its second half: the mean period between upward zero crossings of the
+    mean-free signal (linearly interpolated); NaN without two crossings."""
     x = np.asarray(lift[len(lift) // 2:], dtype=np.float64)
     x = x - x.mean()
-    if len(x) < 8 or not np.any(x):
+    up = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    if len(up) < 2:
         return float("nan")
-    spec = np.abs(np.fft.rfft(x * np.hanning(len(x))))
-    freqs = np.fft.rfftfreq(len(x), d=dt_pu)
-    k = int(np.argmax(spec[1:]) + 1)
-    return float(freqs[k] * char_length_pu / char_velocity_pu)
+    t = up + x[up] / (x[up] - x[up + 1])   # fractional sample index of the crossing
+    period_steps = np.mean(np.diff(t)) * every
+    return float(diameter / (u_lu * period_steps))
 
 
 def cylinder(steps=40000, nx=2048, ny=512, diameter=64, every=50, storage="f32",
@@ -133,22 +136,76 @@ def cylinder(steps=40000, nx=2048, ny=512, diameter=64, every=50, storage="f32",
         cl.append(float(F[1]) / norm)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    dt_pu = flow.units.convert_time_to_pu(every)
+    print(f"cylinder {collision} {storage}: {wall:.1f} s", file=sys.stderr, flush=True)
     half = len(cd) // 2
     return {"study": "cylinder_d2q9", "collision": collision, "storage": storage,
             "grid": [nx, ny], "diameter": diameter, "steps": steps,
             "tau": flow.units.relaxation_parameter_lu,
             "cd_mean": float(np.mean(cd[half:])) if cd else None,
             "cl_amplitude": float(0.5 * (np.max(cl[half:]) - np.min(cl[half:]))) if cl else None,
-            "strouhal": _strouhal(cl, dt_pu),
+            "strouhal": _strouhal(cl, every, diameter, u_lu),
             "finite": bool(np.all(np.isfinite(cd + cl))),
             "wall_s_incl_forces": round(wall, 3),
             "mlups_incl_forces": round(steps * nx * ny / 1e6 / wall, 1)}
 
 
+def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=0):
+    """D3Q27 decaying isotropic turbulence, KBC, E(k) ~ k^4 exp(-2 (k/k0)^2),
+    u_rms = 0.05 and nu = 1e-4 in lattice units; FP32 storage and shifted
+    BF16 / FP16 storage against FP64: kinetic energy, the dissipation rate
+    -dE/dt and nu <|curl u|^2>, every `every` steps."""
+    out = {"study": "turbulence3d_d3q27_kbc", "grid": [res] * 3, "steps": steps,
+           "every": every, "nu_lu": nu, "u_rms_lu": u_rms}
+    # lettuce scales the field to max|u| = Ma cs (decayingturbulence.py:167-169):
+    # pick Ma and Re so that u_rms = 0.05 and nu = 1e-4 in lattice units
+    probe = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float64), [res] * 3, 1.0,
+                                  0.1, k0=k0, stencil=lt.D3Q27(), initialize_pressure=False,
+                                  initialize_fneq=False, randseed=seed)
+    u = probe.u()
+    ratio = float(torch.sqrt((u * u).sum(0).mean()) / torch.linalg.vector_norm(u, dim=0).max())
+    del probe, u
+    torch.cuda.empty_cache()
+    u_max = u_rms / ratio
+    ma = u_max / lt.D3Q27().cs
+    re = res * u_max / nu
+    out["ma"], out["re"] = ma, re
+    series, timing = {}, {}
+    for name, dtype, storage in (("f64", torch.float64, "f64"), ("f32", torch.float32, "f32"),
+                                 ("bf16_shifted", torch.float32, "bf16_shifted"),
+                                 ("f16_shifted", torch.float32, "f16_shifted")):
+        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+        flow = lt.DecayingTurbulence(ctx, [res] * 3, re, ma, k0=k0, stencil=lt.D3Q27(),
+                                     initialize_pressure=False, initialize_fneq=False,
+                                     randseed=seed)
+        sim = lt.Simulation(flow, lt.KBCCollision(), [], lt.StreamingStrategy.PRE_STREAMING)
+        t0 = time.perf_counter()
+        series[name] = _ke_series(sim, flow, steps, every)
+        torch.cuda.synchronize()
+        timing[name] = time.perf_counter() - t0
+        out["tau"] = flow.units.relaxation_parameter_lu
+        out["nu_lu_check"] = flow.units.viscosity_lu
+        print(f"turbulence3d {name}: {timing[name]:.1f} s", file=sys.stderr, flush=True)
+        del sim, flow
+        torch.cuda.empty_cache()
+    ref = series["f64"]
+    for name in ("f32", "bf16_shifted", "f16_shifted"):
+        s = series[name]
+        ke_err = np.abs(s[:, 1] / ref[:, 1] - 1)
+        diss = -np.diff(s[:, 1])
+        diss_ref = -np.diff(ref[:, 1])
+        out[name] = {"ke_rel_err_max": float(ke_err.max()),
+                     "dissipation_rel_err_max": float(np.max(np.abs(diss / diss_ref - 1))),
+                     "enstrophy_rel_err_max": float(np.max(np.abs(s[:, 2] / ref[:, 2] - 1))),
+                     "ke_final_over_initial": float(s[-1, 1] / s[0, 1]),
+                     "wall_s_incl_reporting": round(timing[name], 2)}
+    out["f64"] = {"ke_final_over_initial": float(ref[-1, 1] / ref[0, 1]),
+                  "wall_s_incl_reporting": round(timing["f64"], 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["tgv2d", "tgv3d", "cylinder", "all"])
+    ap.add_argument("study", choices=["tgv2d", "tgv3d", "cylinder", "turbulence3d", "all"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--res", type=int, default=None)
     args = ap.parse_args()
@@ -164,6 +221,9 @@ def main():
             print(json.dumps(cylinder(collision=coll, storage=storage, **kw)), flush=True)
         ref = cylinder(collision="kbc", storage="f64", dtype=torch.float64, **kw)
         print(json.dumps(ref), flush=True)
+    if args.study in ("turbulence3d", "all"):
+        print(json.dumps(turbulence3d(**kw, **({"res": args.res} if args.res else {}))),
+              flush=True)
 
 
 if __name__ == "__main__":
