@@ -244,6 +244,51 @@ int lbm_learned_pack(const float* w1, const float* b1, const float* w2,
                      const float* b2, const float* w3, const float* b3,
                      void* out);
 
+/* ---------------------------------------------------------------------------
+ * Training of the learned collision (no lettuce counterpart: lettuce has no
+ * trainable collision; these are the forward and adjoint kernels behind
+ * lettuce_amd.ext.LearnedRollout, a torch.autograd.Function).
+ *
+ * Periodic D2Q9 with pull streaming, a batch of n[0] independent n[1] x n[2]
+ * fields, populations contiguous as [9][n0][n1][n2] in F32 or F64 (storage),
+ * and the MLP as one vector `params` of the same type:
+ *   [w1 (32x6), b1 (32), w2 (32x32), b2 (32), w3 (3x32), b3 (3)]
+ * The forward step is the learned collision's formula with the MLP in the
+ * arithmetic type (no bf16 rounding) and without the KBC fallback:
+ *   f' = g - ds/tau - sum_k clamp(1/tau + o_k, rate_lo, rate_hi) m_k(dh) phi_k
+ * with g the pulled populations.  The clamp passes the gradient for raw
+ * rates in [rate_lo, rate_hi] and blocks it outside.
+ * ------------------------------------------------------------------------- */
+#define LBM_LEARNED_PARAMS 1379
+/* per-cell record of the backward kernel: x (6), a1 (32), a2 (32),
+ * dL/dz1 (32), dL/dz2 (32), dL/do (3), stored [LBM_LEARNED_RECORD][cells];
+ * the weight gradients are sums over cells of outer products of its rows */
+#define LBM_LEARNED_RECORD 137
+
+/* f_out = learned collide(pull(f_in)) */
+int lbm_learned_train_forward(int32_t storage, const int64_t n[3], double tau,
+                              const void* params, double in_scale,
+                              double rate_lo, double rate_hi,
+                              const void* f_in, void* f_out, void* stream);
+
+/* The adjoint of one step taken from f_in.  lam is the adjoint of that
+ * step's output: read at the cell itself (lam_gathered == 0) or, when lam is
+ * the previous backward call's mu, as lam_q(x) = mu_q(x + e_q)
+ * (lam_gathered != 0), the transpose of pull streaming.  Writes
+ * mu = (d collide / d g)^T lam and, if record is not null, the per-cell
+ * record (LBM_LEARNED_RECORD x cells values of the storage type). */
+int lbm_learned_train_backward(int32_t storage, const int64_t n[3], double tau,
+                               const void* params, double in_scale,
+                               double rate_lo, double rate_hi,
+                               const void* f_in, const void* lam,
+                               int32_t lam_gathered, void* mu, void* record,
+                               void* stream);
+
+/* lam_q(x) = mu_q(x + e_q): the adjoint of periodic pull streaming (the
+ * gradient with respect to the populations before the step) */
+int lbm_unpull(int32_t storage, const int64_t n[3], const void* mu, void* lam,
+               void* stream);
+
 #ifdef __cplusplus
 }
 #endif

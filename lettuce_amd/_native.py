@@ -44,6 +44,8 @@ class LbmCellClass(ctypes.Structure):
 
 LEARNED_PACKED_BYTES = 17408
 LEARNED_HIDDEN = 32
+LEARNED_PARAMS = 1379    # [w1 (32x6), b1, w2 (32x32), b2, w3 (3x32), b3]
+LEARNED_RECORD = 137     # x, a1, a2, dL/dz1, dL/dz2, dL/do per cell
 
 
 class LbmLearned(ctypes.Structure):
@@ -93,6 +95,11 @@ _SIGNATURES = {
                             _vp, _vp, _vp], ctypes.c_int),
     "lbm_learned_pack": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_stream_copy": ([_vp, _vp, _i64, _i32, _vp], ctypes.c_int),
+    "lbm_learned_train_forward": ([_i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp, _vp],
+                                  ctypes.c_int),
+    "lbm_learned_train_backward": ([_i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp, _i32,
+                                    _vp, _vp, _vp], ctypes.c_int),
+    "lbm_unpull": ([_i32, _vp, _vp, _vp, _vp], ctypes.c_int),
 }
 
 _lock = threading.Lock()

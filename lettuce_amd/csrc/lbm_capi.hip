@@ -50,6 +50,11 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace lbm {
+// error reporting for the entry points defined in other translation units
+int report_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace lbm
+
 static int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LBM_E_HIP, "%s: %s", what, hipGetErrorString(e));

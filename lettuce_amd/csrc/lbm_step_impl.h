@@ -527,4 +527,7 @@ KernelFn pick_kernel(int storage, int coll, bool mask, int mode) {
   return nullptr;
 }
 
+// sets the message returned by lbm_last_error and returns code
+int report_error(int code, const char* msg);
+
 }  // namespace lbm

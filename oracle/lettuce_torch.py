@@ -26,6 +26,9 @@ Pinning (what checks each part; see tests/test_oracle.py):
                       wall velocity, exact mass conservation
   masks               golden ncm/nsm from lettuce's Simulation
   TGV / observables   analytic decay, golden vectors
+  learned collision   zero MLP output = BGK, rates 1/tau = BGK, forced
+                      rates against the inference formula; training step's
+                      autograd gradient against central differences
 """
 import math
 from typing import List, Optional
@@ -41,6 +44,8 @@ __all__ = [
     "TaylorGreenVortex", "DecayingTurbulence", "ObstacleChannel",
     "Simulation", "StreamingStrategy", "torch_gradient",
     "incompressible_kinetic_energy", "enstrophy", "mass",
+    "LearnedParameters", "LearnedCollision", "learned_train_collide",
+    "learned_train_step", "split_learned_params",
 ]
 
 
@@ -894,3 +899,60 @@ class LearnedCollision:
         if not fallback.any():
             return f_learned
         return torch.where(fallback[None], self.kbc(flow), f_learned)
+
+
+# --------------------------------------------------------------------------
+# training step of the learned collision: the same formula with the MLP in
+# the populations' dtype (no bf16 rounding) and no KBC fallback, written as
+# differentiable torch so that torch.autograd gives the reference gradients
+# --------------------------------------------------------------------------
+LEARNED_HIDDEN = 32
+LEARNED_PARAMS = 1379
+
+
+def split_learned_params(theta, hidden=LEARNED_HIDDEN):
+    """[w1 (h x 6), b1, w2 (h x h), b2, w3 (3 x h), b3] -> the six tensors"""
+    sizes = [hidden * 6, hidden, hidden * hidden, hidden, 3 * hidden, 3]
+    w1, b1, w2, b2, w3, b3 = torch.split(theta, sizes)
+    return (w1.reshape(hidden, 6), b1, w2.reshape(hidden, hidden), b2,
+            w3.reshape(3, hidden), b3)
+
+
+def learned_train_collide(f, theta, tau, in_scale=100.0, rate_lo=0.05, rate_hi=1.95):
+    """Learned collision of populations f [9, ...] (LearnedCollision's
+    formula) with the MLP evaluated in f's dtype and without the fallback;
+    clamp() passes the gradient inside [rate_lo, rate_hi] only."""
+    dtype = f.dtype
+    e = torch.tensor(D2Q9.e, dtype=dtype)
+    w = torch.tensor(D2Q9.w, dtype=dtype)
+    rho = f.sum(0)
+    u = torch.einsum("qd,q...->d...", e, f) / rho
+    eu = torch.einsum("qd,d...->q...", e, u)
+    uu = (u * u).sum(0)
+    feq = w.reshape([-1] + [1] * u[0].dim()) * rho * (
+        1 + 3 * eu - 1.5 * uu + 4.5 * eu * eu)
+    df = f - feq
+    rows = d2q9_moment_rows(dtype)
+    m = torch.einsum("kq,q...->k...", rows, df)
+    T, N, xy = m[0], m[1], m[2]
+    ds = torch.stack([-T, 0.25 * (T + N), 0.25 * (T - N), 0.25 * (T + N),
+                      0.25 * (T - N), 0.25 * xy, -0.25 * xy, 0.25 * xy, -0.25 * xy])
+    dh = df - ds
+    mg = torch.einsum("kq,q...->k...", rows[3:], dh)
+    w1, b1, w2, b2, w3, b3 = split_learned_params(theta)
+    x = torch.movedim(m / rho * in_scale, 0, -1)          # [..., 6]
+    h = torch.nn.functional.silu(torch.nn.functional.linear(x, w1, b1))
+    h = torch.nn.functional.silu(torch.nn.functional.linear(h, w2, b2))
+    o = torch.movedim(torch.nn.functional.linear(h, w3, b3), -1, 0)
+    rates = (1.0 / tau + o).clamp(rate_lo, rate_hi)
+    phi = torch.tensor([PHI_XXY, PHI_XYY, PHI_XXYY], dtype=dtype)
+    return f - ds / tau - torch.einsum("kq,k...->q...", phi, rates * mg)
+
+
+def learned_train_step(f, theta, tau, **kw):
+    """periodic pull streaming over the last two axes of f [9, ..., X, Y]
+    (lettuce/_simulation.py:241-256 with batch axes in front), then
+    learned_train_collide"""
+    g = torch.stack([torch.roll(f[q], shifts=tuple(D2Q9.e[q]), dims=(-2, -1))
+                     for q in range(9)])
+    return learned_train_collide(g, theta, tau, **kw)

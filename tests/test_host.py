@@ -69,6 +69,48 @@ def test_step_rejects_bad_arguments_without_a_gpu():
     assert lib.lbm_step(ctypes.byref(a), None) == -1
 
 
+def test_training_entry_points_reject_bad_arguments_without_a_gpu():
+    """The training kernels validate storage, extents, tau and buffers
+    before any HIP call."""
+    lib = N.lib()
+    n = (ctypes.c_int64 * 3)(2, 8, 8)
+    p, a, b = ctypes.c_void_p(4096), ctypes.c_void_p(8192), ctypes.c_void_p(16384)
+    fwd = lib.lbm_learned_train_forward
+    assert fwd(N.F16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -3
+    assert fwd(N.F32, n, 0.5, p, 100.0, 0.05, 1.95, a, b, None) == -1
+    assert b"tau" in lib.lbm_last_error()
+    assert fwd(N.F32, n, 0.6, p, 100.0, 0.05, 1.95, a, a, None) == -1
+    assert fwd(N.F32, n, 0.6, None, 100.0, 0.05, 1.95, a, b, None) == -1
+    assert fwd(N.F32, n, 0.6, p, 100.0, 1.5, 1.0, a, b, None) == -1
+    zero = (ctypes.c_int64 * 3)(2, 0, 8)
+    assert fwd(N.F64, zero, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -1
+    bwd = lib.lbm_learned_train_backward
+    assert bwd(N.F64, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, b, None, None) == -1
+    assert bwd(N.BF16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, p, None, None) == -3
+    assert lib.lbm_unpull(N.F32, n, a, a, None) == -1
+    big = (ctypes.c_int64 * 3)(1 << 20, 1 << 10, 1 << 10)
+    assert lib.lbm_unpull(N.F32, big, a, b, None) == -1
+
+
+def test_training_rollout_needs_a_device_tensor():
+    f = torch.zeros(9, 2, 4, 4)
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        lt.learned_rollout(f, torch.zeros(N.LEARNED_PARAMS), lt.TrainConfig(tau=0.6), 2)
+
+
+def test_flat_parameters_roundtrip():
+    p = lt.LearnedParameters.seeded(4, in_scale=50.0)
+    theta = lt.flatten_parameters(p)
+    assert theta.numel() == N.LEARNED_PARAMS
+    q = lt.unflatten_parameters(theta, like=p)
+    for a, b in zip(p.tensors(), q.tensors()):
+        assert torch.equal(a, b)
+    assert q.in_scale == 50.0
+    # the oracle splits the same vector into the same tensors
+    for a, b in zip(p.tensors(), O.split_learned_params(theta)):
+        assert torch.equal(a, b)
+
+
 def _cpu_flow(stencil, res):
     return SimpleNamespace(stencil=stencil, resolution=list(res),
                            context=lt.Context("cpu"), torch_stencil=None)
