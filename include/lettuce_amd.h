@@ -209,6 +209,16 @@ int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3],
                    const uint8_t* mask, double* out, void* workspace,
                    void* stream);
 
+/* Failure scan over all cells, the checks of NaNReporter.is_failed
+ * (torch.isnan(flow.f).any(), lettuce/ext/_reporter/failure_reporter.py:144-145)
+ * and HighMaReporter.is_failed (|u| / cs > threshold, :168-171) in one pass:
+ * out[0..2] = { number of NaN populations, number of cells whose |u|^2 is
+ * not finite, max |u|^2 over the other cells } (FP64, deterministic).
+ * `workspace` holds at least lbm_reduce_workspace_bytes(n) bytes. */
+int lbm_scan_f64(int32_t stencil, int32_t storage, const int64_t n[3],
+                 const int64_t stride[4], const void* f, double rho0_shift,
+                 double* out, void* workspace, void* stream);
+
 /* FP64 enstrophy sum, sum over cells of |curl u|^2 with u in lattice units
  * and 6th-order periodic central differences (dx = 1), the sum computed by
  * Enstrophy (observable_reporter.py:45-68) via torch_gradient

@@ -91,6 +91,7 @@ _SIGNATURES = {
     "lbm_reduce_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _vp, _vp,
                         _vp], ctypes.c_int),
     "lbm_enstrophy_f64": ([_i32, _I64_3, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_scan_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _vp, _vp], ctypes.c_int),
     "lbm_link_force_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _i64,
                             _vp, _vp, _vp], ctypes.c_int),
     "lbm_learned_pack": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),

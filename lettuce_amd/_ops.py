@@ -133,6 +133,19 @@ def reduce_f64(stencil, f: torch.Tensor, rho0: float = 1.0,
     return out
 
 
+def scan_f64(stencil, f: torch.Tensor, rho0: float = 1.0) -> torch.Tensor:
+    """FP64 [NaN populations, cells with non-finite |u|^2, max finite |u|^2]
+    over all cells, in one pass over the stored populations."""
+    N.require_device_tensor(f, "f")
+    n, s = layout(f, stencil.q, stencil.d)
+    out = torch.empty(3, dtype=torch.float64, device=f.device)
+    ws = _workspace(f.device, n)
+    N.check(N.lib().lbm_scan_f64(
+        stencil.native_id, STORAGE_OF[f.dtype], _n(n), _s(s), f.data_ptr(), rho0,
+        out.data_ptr(), ws.data_ptr(), N.stream_handle(f.device)), "lbm_scan_f64")
+    return out
+
+
 def enstrophy_f64(stencil, u: torch.Tensor) -> torch.Tensor:
     """FP64 sum of |curl u|^2 (lattice units, 6th-order periodic FD)."""
     res = tuple(u.shape[1:])

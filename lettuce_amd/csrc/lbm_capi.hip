@@ -467,6 +467,62 @@ __global__ void k_finish(const double* part, int nparts, double* out, int nout) 
     for (int k = 0; k < nout; ++k) out[k] = acc[k];
 }
 
+// Failure scan: part[b] = { NaN populations, cells with non-finite u,
+// max |u|^2 over the cells with finite u } of block b
+template <class S, class ST>
+__global__ void __launch_bounds__(RED_BLOCK) k_scan(Grid g, const void* f, double rho0,
+                                                    double* part) {
+  using C = typename ST::C;
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  double umax = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < g.cells;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    C v[S::Q];
+    load_cell<S, ST>(g, f, cell_offset(g, c), rho0, v);
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q)
+      if (v[q] != v[q]) acc[0] += 1.0;
+    double r, j[3];
+    moments_f64<S, C>(v, r, j);
+    const double uu = (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]) / (r * r);
+    if (isfinite(uu)) umax = fmax(umax, uu);
+    else acc[1] += 1.0;
+  }
+  acc[2] = umax;
+  __shared__ double shmax[RED_BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) umax = fmax(umax, __shfl_xor(umax, o, 64));
+  if ((threadIdx.x & 63) == 0) shmax[threadIdx.x >> 6] = umax;
+  block_sum(acc);
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int w = 0; w < RED_BLOCK / 64; ++w) m = fmax(m, shmax[w]);
+    part[blockIdx.x * RED_N + 0] = acc[0];
+    part[blockIdx.x * RED_N + 1] = acc[1];
+    part[blockIdx.x * RED_N + 2] = m;
+  }
+}
+
+// counts summed, maxima maxed, in a fixed order (deterministic)
+__global__ void k_finish_scan(const double* part, int nparts, double* out) {
+  double acc[RED_N] = {0, 0, 0, 0, 0};
+  double m = 0;
+  for (int b = threadIdx.x; b < nparts; b += RED_BLOCK) {
+    acc[0] += part[b * RED_N + 0];
+    acc[1] += part[b * RED_N + 1];
+    m = fmax(m, part[b * RED_N + 2]);
+  }
+  __shared__ double shmax[RED_BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) shmax[threadIdx.x >> 6] = m;
+  block_sum(acc);
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < RED_BLOCK / 64; ++w) m = fmax(m, shmax[w]);
+    out[0] = acc[0];
+    out[1] = acc[1];
+    out[2] = m;
+  }
+}
+
 // 6th-order periodic central difference along axis `a` (weights of
 // torch_gradient order 6, lettuce/util/utility.py:52-56)
 __device__ __forceinline__ double d6(const double* u, int64_t i0, int64_t i1, int64_t i2,
@@ -637,6 +693,16 @@ struct RedRun {
 };
 
 template <class S, class ST>
+struct ScanRun {
+  static int run(Grid g, const void* f, double rho0, double* out, double* ws, hipStream_t s) {
+    const unsigned nb = grid_blocks(g.cells, RED_BLOCK, RED_MAX_BLOCKS);
+    hipLaunchKernelGGL((k_scan<S, ST>), dim3(nb), dim3(RED_BLOCK), 0, s, g, f, rho0, ws);
+    hipLaunchKernelGGL(k_finish_scan, dim3(1), dim3(RED_BLOCK), 0, s, (const double*)ws, (int)nb, out);
+    return check_launch("lbm_scan_f64");
+  }
+};
+
+template <class S, class ST>
 struct ForceRun {
   static int run(Grid g, const void* f, double rho0, const int32_t* links, int64_t nl, double* out,
                  double* ws, hipStream_t s) {
@@ -705,6 +771,16 @@ int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3], const i
   return with_lattice<RedRun>(stencil, storage, make_grid(n, stride), f,
                               shift_for(storage, rho0_shift), mask, out, (double*)workspace,
                               (hipStream_t)stream);
+}
+
+int lbm_scan_f64(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                 const void* f, double rho0_shift, double* out, void* workspace, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !out || !workspace) return fail(LBM_E_ARG, "NULL pointer");
+  return with_lattice<ScanRun>(stencil, storage, make_grid(n, stride), f,
+                               shift_for(storage, rho0_shift), out, (double*)workspace,
+                               (hipStream_t)stream);
 }
 
 int lbm_enstrophy_f64(int32_t stencil, const int64_t n[3], const double* u, double* out,

@@ -4,3 +4,4 @@ from ._learned import *  # noqa: F401,F403
 from ._training import *  # noqa: F401,F403
 from ._flows import *  # noqa: F401,F403
 from ._reporter import *  # noqa: F401,F403
+from ._failure import *  # noqa: F401,F403

@@ -92,6 +92,20 @@ def test_training_entry_points_reject_bad_arguments_without_a_gpu():
     assert lib.lbm_unpull(N.F32, big, a, b, None) == -1
 
 
+def test_scan_rejects_bad_arguments_without_a_gpu():
+    lib = N.lib()
+    n = (ctypes.c_int64 * 3)(1, 8, 8)
+    s = (ctypes.c_int64 * 4)(64, 0, 8, 1)
+    out = (ctypes.c_double * 3)()
+    ws = ctypes.c_void_p(4096)
+    assert lib.lbm_scan_f64(N.D2Q9, N.F32, n, s, ctypes.c_void_p(8192), 1.0, out, None,
+                            None) == -1
+    assert lib.lbm_scan_f64(9, N.F32, n, s, ctypes.c_void_p(8192), 1.0, out, ws, None) == -1
+    bad = (ctypes.c_int64 * 4)(8, 0, 8, 1)    # populations overlap
+    assert lib.lbm_scan_f64(N.D2Q9, N.F32, n, bad, ctypes.c_void_p(8192), 1.0, out, ws,
+                            None) == -1
+
+
 def test_training_rollout_needs_a_device_tensor():
     f = torch.zeros(9, 2, 4, 4)
     with pytest.raises(RuntimeError, match="ROCm GPU"):
