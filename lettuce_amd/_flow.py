@@ -222,3 +222,74 @@ class Flow(ABC):
             self.f = self.context.convert_to_tensor(
                 np.load(fh, allow_pickle=False), dtype=self.context.dtype)
         self._state_next = None
+
+    _UNIT_FIELDS = ("reynolds_number", "mach_number", "characteristic_length_pu",
+                    "characteristic_velocity_pu", "characteristic_length_lu",
+                    "characteristic_density_lu", "characteristic_density_pu", "cs")
+
+    def save_checkpoint(self, filename):
+        """Restart file (.npz, no pickle): the stored populations bit for bit
+        in their storage format, the step counter flow.i, the stencil, the
+        storage format and the unit parameters.  A superset of dump(), which
+        keeps only the decoded populations (lettuce/_flow.py:258-268)."""
+        state = self._state.detach()
+        if state.dtype in (torch.float16, torch.bfloat16):
+            raw = state.view(torch.int16)
+        else:
+            raw = state
+        perm = [0] + list(range(1, state.dim()))
+        if self.context.layout == "rows":
+            # memory order [*res[:-1], Q, res[-1]]: save as stored
+            perm = list(range(1, state.dim() - 1)) + [0, state.dim() - 1]
+        raw = raw.permute(perm).contiguous().cpu().numpy()
+        meta = {"format": np.array("lettuce_amd-checkpoint-1"),
+                "stencil": np.array(type(self.stencil).__name__),
+                "storage": np.array(self.context.storage),
+                "layout": np.array(self.context.layout),
+                "state_dtype": np.array(str(state.dtype).replace("torch.", "")),
+                "i": np.array(self.i, dtype=np.int64),
+                "resolution": np.array(self.resolution, dtype=np.int64),
+                "rho0_shift": np.array(self.rho0_shift, dtype=np.float64)}
+        for k in self._UNIT_FIELDS:
+            meta["unit_" + k] = np.array(float(getattr(self.units, k)), dtype=np.float64)
+        with open(filename, "wb") as fh:
+            np.savez(fh, state=raw, **meta)
+
+    def load_checkpoint(self, filename, check_units: bool = True):
+        """Restore a save_checkpoint() file into this flow: the stored
+        populations bit for bit and flow.i.  The stencil, resolution,
+        storage format and layout must match this flow's context; with
+        check_units the unit parameters must match too."""
+        with np.load(filename, allow_pickle=False) as z:
+            if str(z["format"]) != "lettuce_amd-checkpoint-1":
+                raise ValueError(f"{filename} is not a lettuce_amd checkpoint")
+            for key, mine in (("stencil", type(self.stencil).__name__),
+                              ("storage", self.context.storage),
+                              ("layout", self.context.layout)):
+                if str(z[key]) != mine:
+                    raise ValueError(f"checkpoint {key} {str(z[key])!r} differs from "
+                                     f"this flow's {mine!r}")
+            if list(z["resolution"]) != list(self.resolution):
+                raise ValueError(f"checkpoint resolution {list(z['resolution'])} differs "
+                                 f"from {self.resolution}")
+            if check_units:
+                for k in self._UNIT_FIELDS:
+                    if float(z["unit_" + k]) != float(getattr(self.units, k)):
+                        raise ValueError(f"checkpoint unit {k} = {float(z['unit_' + k])} "
+                                         f"differs from {getattr(self.units, k)}")
+            raw = torch.from_numpy(np.ascontiguousarray(z["state"]))
+            i = int(z["i"])
+        state = self.context.population_tensor(self.stencil.q, self.resolution)
+        if state.dtype in (torch.float16, torch.bfloat16):
+            target = state.view(torch.int16)
+        else:
+            target = state
+        if raw.dtype != target.dtype:
+            raise ValueError(f"checkpoint holds {raw.dtype}, this flow stores {target.dtype}")
+        perm = [0] + list(range(1, state.dim()))
+        if self.context.layout == "rows":
+            perm = list(range(1, state.dim() - 1)) + [0, state.dim() - 1]
+        target.permute(perm).copy_(raw.to(target.device))
+        self._state = state
+        self._state_next = None
+        self.i = i

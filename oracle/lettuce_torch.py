@@ -40,7 +40,7 @@ __all__ = [
     "Context", "D2Q9", "D3Q19", "D3Q27", "UnitConversion",
     "QuadraticEquilibrium", "NoCollision", "BGKCollision", "TRTCollision",
     "KBCCollision", "BounceBackBoundary", "EquilibriumBoundaryPU",
-    "EquilibriumOutletP", "HalfwayBounceBackBoundary", "Flow",
+    "EquilibriumOutletP", "AntiBounceBackOutlet", "HalfwayBounceBackBoundary", "Flow",
     "TaylorGreenVortex", "DecayingTurbulence", "ObstacleChannel",
     "Simulation", "StreamingStrategy", "torch_gradient",
     "incompressible_kinetic_energy", "enstrophy", "mass",
@@ -445,6 +445,57 @@ class EquilibriumOutletP:  # equilibrium_outlet_p.py:12-91
     def make_no_streaming_mask(self, shape, context):
         m = torch.zeros(shape, dtype=torch.bool)
         m[tuple([np.setdiff1d(np.arange(shape[0]), self.velocities)] + self.index)] = 1
+        return m
+
+    def make_no_collision_mask(self, shape, context):
+        m = torch.zeros(shape, dtype=torch.bool)
+        m[tuple(self.index)] = 1
+        return m
+
+
+class AntiBounceBackOutlet:  # anti_bounce_back_outlet.py:13-109
+    """f_opp(v) <- -f_v + w_v rho (2 + (e_v.u_w)^2 / cs^4 - |u_w|^2 / cs^2) on
+    the outlet column for the velocities v leaving the domain, with
+    u_w = u + (u - u_neighbour) / 2 (:79-92)"""
+
+    def __init__(self, direction, flow):
+        direction = np.array(direction)
+        self.velocities = np.concatenate(np.argwhere(
+            np.matmul(flow.stencil.e, direction) > 1 - 1e-6), axis=0)
+        self.opposite = np.array(flow.stencil.opposite)
+        self.index, self.neighbor = [], []
+        for c in direction:
+            if c == 0:
+                self.index.append(slice(None))
+                self.neighbor.append(slice(None))
+            if c == 1:
+                self.index.append(-1)
+                self.neighbor.append(-2)
+            if c == -1:
+                self.index.append(0)
+                self.neighbor.append(1)
+        w = torch.tensor(flow.stencil.w, dtype=flow.context.dtype)[self.velocities]
+        nd = len(direction)
+        self.w = w.reshape([-1] + [1] * (nd - 1))
+        self.e = torch.tensor(flow.stencil.e, dtype=flow.context.dtype)[self.velocities]
+
+    def __call__(self, flow):
+        here = tuple([slice(None)] + self.index)
+        other = tuple([slice(None)] + self.neighbor)
+        u = flow.u()
+        u_w = u[here] + 0.5 * (u[here] - u[other])
+        cs = flow.stencil.cs
+        f = flow.f
+        eu = torch.einsum("dc,c...->d...", self.e, u_w)
+        f[tuple([self.opposite[self.velocities]] + self.index)] = (
+            -flow.f[tuple([self.velocities] + self.index)]
+            + self.w * flow.rho()[here]
+            * (2 + eu ** 2 / cs ** 4 - (torch.norm(u_w, dim=0) / cs) ** 2))
+        return f
+
+    def make_no_streaming_mask(self, shape, context):
+        m = torch.zeros(shape, dtype=torch.bool)
+        m[tuple([self.opposite[self.velocities]] + self.index)] = 1
         return m
 
     def make_no_collision_mask(self, shape, context):

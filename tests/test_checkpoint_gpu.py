@@ -1,0 +1,71 @@
+"""Checkpoints: lettuce's dump/load test (tests/test_checkpoint.py:5-20) and
+the bit-exact restart file (Flow.save_checkpoint / load_checkpoint)."""
+import pytest
+import torch
+
+import lettuce_amd as lt
+
+pytestmark = pytest.mark.gpu
+
+
+def _flow(storage="f32", layout="soa", d=2, res=32):
+    dtype = torch.float64 if storage == "f64" else torch.float32
+    ctx = lt.Context("cuda:0", dtype=dtype, storage=storage, layout=layout)
+    stencil = lt.D2Q9() if d == 2 else lt.D3Q19()
+    return lt.TaylorGreenVortex(ctx, res, 100, 0.1, stencil, initialize_fneq=False)
+
+
+def _bits(t):
+    """the stored bytes in logical [q, *res] order"""
+    return t.detach().cpu().contiguous().view(-1).view(torch.uint8)
+
+
+def _sim(flow):
+    return lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                         lt.StreamingStrategy.PRE_STREAMING)
+
+
+def test_dump_load_restores_f(tmp_path):
+    flow = _flow()
+    f0 = flow.f.clone()
+    flow.dump(tmp_path / "tgv")
+    _sim(flow)(10)
+    assert not torch.equal(flow.f, f0)
+    flow.load(tmp_path / "tgv")
+    assert torch.equal(flow.f, f0)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16_shifted", "bf16_shifted"])
+@pytest.mark.parametrize("layout,d", [("soa", 2), ("rows", 3)])
+def test_restart_is_bit_exact(tmp_path, storage, layout, d):
+    res = 32 if d == 2 else 16
+    a = _flow(storage, layout, d, res)
+    sim = _sim(a)
+    sim(7)
+    path = tmp_path / "ck.npz"
+    a.save_checkpoint(path)
+    saved = a._state.clone()
+    sim(5)
+    b = _flow(storage, layout, d, res)
+    b.load_checkpoint(path)
+    assert b.i == 7
+    assert b._state.stride() == saved.stride()
+    assert torch.equal(_bits(b._state), _bits(saved))
+    _sim(b)(5)
+    assert b.i == a.i == 12
+    assert torch.equal(_bits(b._state), _bits(a._state))
+
+
+def test_checkpoint_mismatches_are_rejected(tmp_path):
+    a = _flow("f32")
+    path = tmp_path / "ck.npz"
+    a.save_checkpoint(path)
+    with pytest.raises(ValueError, match="storage"):
+        _flow("f16_shifted").load_checkpoint(path)
+    with pytest.raises(ValueError, match="resolution"):
+        _flow("f32", res=16).load_checkpoint(path)
+    other = lt.TaylorGreenVortex(lt.Context("cuda:0"), 32, 200, 0.1, lt.D2Q9())
+    with pytest.raises(ValueError, match="reynolds_number"):
+        other.load_checkpoint(path)
+    other.load_checkpoint(path, check_units=False)
+    assert other.i == 0 and torch.equal(other.f, a.f)
