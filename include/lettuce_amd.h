@@ -56,13 +56,19 @@ enum lbm_storage {
  *  KBC     lettuce/ext/_collision/kbc_collision.py:96-160 (D2Q9, D3Q27)
  *  LEARNED KBC decomposition whose higher-order rates come from a per-cell
  *          MLP (no lettuce counterpart; nearest hook MRTCollision,
- *          lettuce/ext/_collision/mrt_collision.py:8-35), D2Q9 only. */
+ *          lettuce/ext/_collision/mrt_collision.py:8-35), D2Q9 only.
+ *  BGK_GUO BGK with Guo forcing by a uniform acceleration a
+ *          (bgk_collision.py:17-22 with ext/_force/guo.py:15-36):
+ *          u* = j / rho + a / (2 rho),
+ *          f' = f - (f - feq(rho, u*)) / tau
+ *               + (1 - 1 / (2 force_tau)) w (3 (e - u*) + 9 (e.u*) e) . a */
 enum lbm_collision {
   LBM_COLL_NONE = 0,
   LBM_COLL_BGK = 1,
   LBM_COLL_TRT = 2,
   LBM_COLL_KBC = 3,
-  LBM_COLL_LEARNED = 4
+  LBM_COLL_LEARNED = 4,
+  LBM_COLL_BGK_GUO = 5
 };
 
 /* streaming strategy, same bit values as lettuce's StreamingStrategy
@@ -82,7 +88,10 @@ enum lbm_cell_op {
   LBM_OP_BOUNCE_BACK = 1, /* f_q <- f_opp(q)      bounce_back_boundary.py:17 */
   LBM_OP_EQUILIBRIUM = 2, /* f <- feq(rho, u)     equilibrium_boundary_pu.py:79-84 */
   LBM_OP_OUTLET = 3,      /* f <- feq(rho, u(x - dir)) equilibrium_outlet_p.py:63-73 */
-  LBM_OP_KEEP = 4         /* f unchanged (no transformer, halfway-BB solid) */
+  LBM_OP_KEEP = 4,        /* f unchanged (no transformer, halfway-BB solid) */
+  LBM_OP_ANTI_BOUNCE_BACK = 5 /* f_opp(v) <- -f_v + w_v rho (2 + 9 (e_v.u_w)^2
+                                 - 3 |u_w|^2), u_w = u + (u - u(x - dir)) / 2,
+                                 for e_v.dir = 1; anti_bounce_back_outlet.py:79-92 */
 };
 
 /* cell-class flags */
@@ -95,7 +104,8 @@ enum lbm_cell_op {
 typedef struct lbm_cell_class {
   uint8_t op;         /* enum lbm_cell_op                                      */
   uint8_t flags;      /* LBM_CLASS_*                                           */
-  int8_t dir[3];      /* OUTLET: the velocity is taken at x - dir (i0,i1,i2)   */
+  int8_t dir[3];      /* OUTLET, ANTI_BOUNCE_BACK: outward normal (i0,i1,i2);
+                         the neighbour velocity is taken at x - dir          */
   uint8_t pad[3];
   uint32_t no_stream; /* bit q set: population q is not streamed into the cell */
   uint32_t pad2;
@@ -142,8 +152,8 @@ typedef struct lbm_step_args {
   double tau_minus;    /* TRT tau_minus                                       */
   double rho0_shift;   /* shifted storage reference density (usually 1)       */
   const lbm_learned* learned; /* host pointer to parameters, LEARNED only      */
-  int64_t fallback_count_ptr_reserved; /* must be 0                           */
-  double reserved[3];
+  double force_tau;    /* BGK_GUO: tau of the source term (Guo's own tau)      */
+  double accel[3];     /* BGK_GUO: acceleration in lattice units (x, y, z)     */
 } lbm_step_args;
 
 /* ABI version of the loaded library (LBM_ABI_VERSION it was built with). */

@@ -193,12 +193,20 @@ class Flow(ABC):
 
     def u(self, f: Optional[torch.Tensor] = None, rho=None,
           acceleration=None) -> torch.Tensor:
-        if acceleration is not None:
-            raise NotImplementedError("forcing is not supported natively")
+        """velocity; with `acceleration` the forcing correction a / (2 rho)
+        is added (lettuce/_flow.py:178-193)"""
         if rho is None:
-            _, u = self._moments(f)
-            return u.to(self.context.dtype)
-        return self.j(f) / rho
+            rho_f64, u = self._moments(f)
+            u = u.to(self.context.dtype)
+            rho = rho_f64[None].to(self.context.dtype)
+        else:
+            u = self.j(f) / rho
+        if acceleration is None:
+            return u
+        a = self.context.convert_to_tensor(acceleration)
+        if a.dim() == 1:
+            a = a[(Ellipsis,) + (None,) * self.stencil.d]
+        return u + a / (2 * rho)
 
     @property
     def velocity(self):

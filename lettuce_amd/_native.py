@@ -26,8 +26,9 @@ ABI_VERSION = 1
 # enum values of include/lettuce_amd.h
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
 F32, F64, F16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3
-COLL_NONE, COLL_BGK, COLL_TRT, COLL_KBC, COLL_LEARNED = 0, 1, 2, 3, 4
+COLL_NONE, COLL_BGK, COLL_TRT, COLL_KBC, COLL_LEARNED, COLL_BGK_GUO = 0, 1, 2, 3, 4, 5
 OP_COLLIDE, OP_BOUNCE_BACK, OP_EQUILIBRIUM, OP_OUTLET, OP_KEEP = 0, 1, 2, 3, 4
+OP_ANTI_BOUNCE_BACK = 5
 CLASS_SOLID, CLASS_NEAR_SOLID, CLASS_BC_FIELD = 0x01, 0x02, 0x04
 
 STENCIL_IDS = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
@@ -65,8 +66,8 @@ class LbmStepArgs(ctypes.Structure):
                 ("bc_field", ctypes.c_void_p), ("tau", ctypes.c_double),
                 ("tau_minus", ctypes.c_double), ("rho0_shift", ctypes.c_double),
                 ("learned", ctypes.POINTER(LbmLearned)),
-                ("fallback_count_ptr_reserved", ctypes.c_int64),
-                ("reserved", ctypes.c_double * 3)]
+                ("force_tau", ctypes.c_double),
+                ("accel", ctypes.c_double * 3)]
 
 
 _I64_3 = ctypes.c_int64 * 3

@@ -97,6 +97,8 @@ class NativeStep:
         a.streaming = simulation.streaming_strategy.value
         a.tau = desc.tau
         a.tau_minus = desc.tau_minus
+        a.force_tau = desc.force_tau
+        a.accel[:] = list(desc.accel) + [0.0] * (3 - len(desc.accel))
         a.rho0_shift = flow.rho0_shift
         self._learned = desc.learned_struct()
         if self._learned is not None:
@@ -210,29 +212,30 @@ class Simulation:
         return desc.op_spec(self.flow)
 
     def _check_outlets(self, ops):
-        """The equilibrium outlet writes its whole column in place
-        (lettuce/ext/_boundary/equilibrium_outlet_p.py:69-72) and reads the
-        velocity of the neighbouring column; both are reproduced only when
-        no later boundary claims part of the column and the neighbours
-        collide with a momentum-conserving operator or keep their values."""
+        """The outlets write their whole column in place
+        (lettuce/ext/_boundary/equilibrium_outlet_p.py:69-72,
+        anti_bounce_back_outlet.py:85-92) and read the velocity of the
+        neighbouring column; both are reproduced only when no later boundary
+        claims part of the column and the neighbours collide with a
+        momentum-conserving operator or keep their values."""
         ncm = self.no_collision_mask
         for i, spec in ops.items():
-            if spec.op != N.OP_OUTLET:
+            if spec.op not in (N.OP_OUTLET, N.OP_ANTI_BOUNCE_BACK):
                 continue
             column = spec.column_mask
             if not bool((ncm[column] == i).all()):
                 raise NotImplementedError(
-                    "an equilibrium outlet column overlapped by a later "
+                    "an outlet column overlapped by a later "
                     "boundary is not supported natively")
             # ncm of the neighbour x - direction, seen from x
             nb_idx = torch.roll(ncm, shifts=tuple(spec.lattice_direction),
                                 dims=tuple(range(ncm.dim())))[column]
             for idx in torch.unique(nb_idx).tolist():
                 nb_op = ops.get(idx, OpSpec()).op
-                if nb_op not in (N.OP_COLLIDE, N.OP_KEEP, N.OP_OUTLET):
+                if nb_op not in (N.OP_COLLIDE, N.OP_KEEP, N.OP_OUTLET,
+                                 N.OP_ANTI_BOUNCE_BACK):
                     raise NotImplementedError(
-                        "the equilibrium outlet's neighbour column must be "
-                        "fluid or solid cells")
+                        "an outlet's neighbour column must be fluid or solid cells")
 
     # -- driver -------------------------------------------------------------
     @property

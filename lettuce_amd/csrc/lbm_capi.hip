@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <cmath>
 #include <type_traits>
 
 #include "lbm_step_impl.h"
@@ -160,15 +161,22 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   if (!a->f_in || !a->f_out) return fail(LBM_E_ARG, "NULL population pointer");
   if (a->f_in == a->f_out) return fail(LBM_E_ARG, "f_in and f_out must be distinct buffers");
   if (a->streaming < 0 || a->streaming > 3) return fail(LBM_E_ARG, "unknown streaming %d", a->streaming);
-  if (a->collision < 0 || a->collision > LBM_COLL_LEARNED)
+  if (a->collision < 0 || a->collision > LBM_COLL_BGK_GUO)
     return fail(LBM_E_ARG, "unknown collision %d", a->collision);
   const bool mask = a->cell_class != nullptr;
   if (mask && (!a->classes || a->n_classes < 1 || a->n_classes > 256))
     return fail(LBM_E_ARG, "cell_class given without a class table of 1..256 entries");
   if (mask && !contiguous_cells(a->n, a->stride))
     return fail(LBM_E_UNSUPPORTED, "boundaries need each population stored as one contiguous block");
-  if (a->fallback_count_ptr_reserved != 0) return fail(LBM_E_ARG, "reserved field must be 0");
   if (a->collision == LBM_COLL_BGK && !(a->tau > 0)) return fail(LBM_E_ARG, "BGK needs tau > 0");
+  if (a->collision == LBM_COLL_BGK_GUO) {
+    if (!(a->tau > 0) || !(a->force_tau > 0))
+      return fail(LBM_E_ARG, "Guo forcing needs tau > 0 and force_tau > 0");
+    for (int d = 0; d < 3; ++d)
+      if (!std::isfinite(a->accel[d])) return fail(LBM_E_ARG, "acceleration must be finite");
+    if (a->stencil == LBM_D2Q9 && a->accel[2] != 0)
+      return fail(LBM_E_ARG, "a 2D acceleration has no z component");
+  }
   if (a->collision == LBM_COLL_TRT && !(a->tau > 0 && a->tau_minus > 0))
     return fail(LBM_E_ARG, "TRT needs tau > 0 and tau_minus > 0");
   if (a->collision == LBM_COLL_KBC && !(a->tau > 0)) return fail(LBM_E_ARG, "KBC needs tau > 0");
@@ -247,7 +255,10 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.nblocks = (int)nb;
   p.order = block_order(p, nb);
   const bool learned = a->collision == LBM_COLL_LEARNED;
-  p.tau_inv = (a->collision == LBM_COLL_BGK || learned) ? 1.0 / a->tau : 0.0;
+  const bool guo = a->collision == LBM_COLL_BGK_GUO;
+  p.tau_inv = (a->collision == LBM_COLL_BGK || learned || guo) ? 1.0 / a->tau : 0.0;
+  for (int d = 0; d < 3; ++d) p.accel[d] = guo ? a->accel[d] : 0.0;
+  p.guo = guo ? 1.0 - 1.0 / (2.0 * a->force_tau) : 0.0;
   p.trt_p = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau) : 0.0;
   p.trt_m = a->collision == LBM_COLL_TRT ? 1.0 / (2.0 * a->tau_minus) : 0.0;
   p.beta = (a->collision == LBM_COLL_KBC || learned) ? 1.0 / (2.0 * a->tau) : 0.0;

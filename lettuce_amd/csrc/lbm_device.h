@@ -191,6 +191,8 @@ struct CollParams {
   C trt_m;        // TRT: 1 / (2 tau_minus)
   C beta;         // KBC: 1 / (2 tau)
   C beta_inv;     // KBC: 2 tau
+  C accel[3];     // BGK_GUO: acceleration (lattice units)
+  C guo;          // BGK_GUO: 1 - 1 / (2 force_tau)
 };
 
 // BGK: f - (f - feq) / tau  (lettuce/ext/_collision/bgk_collision.py:17-22)
@@ -203,6 +205,28 @@ __device__ __forceinline__ void collide_bgk(C (&f)[S::Q], const CollParams<C>& c
   for (int q = 0; q < S::Q; ++q) {
     const C fe = feq_q<S, C>(q, rho, u, uu);
     f[q] = f[q] - cp.tau_inv * (f[q] - fe);
+  }
+}
+
+// BGK with Guo forcing (lettuce/ext/_collision/bgk_collision.py:17-22,
+// lettuce/ext/_force/guo.py:15-36): u* = j / rho + a / (2 rho),
+// f - (f - feq(rho, u*)) / tau + guo w_q (3 (e_q - u*) + 9 (e_q.u*) e_q) . a
+template <class S, class C>
+__device__ __forceinline__ void collide_bgk_guo(C (&f)[S::Q], const CollParams<C>& cp) {
+  C rho, u[3];
+  moments<S, C>(f, rho, u);
+  const C half_ri = C(0.5) / rho;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u[d] += cp.accel[d] * half_ri;
+  const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const C ua = u[0] * cp.accel[0] + u[1] * cp.accel[1] + u[2] * cp.accel[2];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const C fe = feq_q<S, C>(q, rho, u, uu);
+    const C eu = eu_of<S, C>(q, u);
+    const C ea = eu_of<S, C>(q, cp.accel);
+    const C si = cp.guo * C(S::w[q]) * (C(3) * (ea - ua) + C(9) * eu * ea);
+    f[q] = f[q] - cp.tau_inv * (f[q] - fe) + si;
   }
 }
 

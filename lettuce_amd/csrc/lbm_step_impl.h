@@ -33,6 +33,7 @@ struct KParams {
   int order;          // block order, one of BlockOrder
   double tau_inv, trt_p, trt_m, beta, beta_inv;
   double rho0;
+  double accel[3], guo;  // BGK_GUO
 };
 
 // Which row (i0, i1) and which run b2 of the row a block takes.  The
@@ -70,6 +71,10 @@ __device__ __forceinline__ CollParams<C> coll_params(const KParams& p) {
   cp.trt_m = (C)p.trt_m;
   cp.beta = (C)p.beta;
   cp.beta_inv = (C)p.beta_inv;
+  cp.accel[0] = (C)p.accel[0];
+  cp.accel[1] = (C)p.accel[1];
+  cp.accel[2] = (C)p.accel[2];
+  cp.guo = (C)p.guo;
   return cp;
 }
 
@@ -143,11 +148,37 @@ __device__ __forceinline__ void collide(C (&f)[S::Q], const CollParams<C>& cp) {
   if constexpr (COLL == LBM_COLL_BGK) collide_bgk<S, C>(f, cp);
   else if constexpr (COLL == LBM_COLL_TRT) collide_trt<S, C>(f, cp);
   else if constexpr (COLL == LBM_COLL_KBC) collide_kbc<S, C>(f, cp);
+  else if constexpr (COLL == LBM_COLL_BGK_GUO) collide_bgk_guo<S, C>(f, cp);
   // LBM_COLL_NONE: f unchanged
 }
 
+// Populations of the neighbour x - dir of an outlet cell as the streaming
+// step leaves them (its moments are those of its post-collision state when
+// it collides with a momentum-conserving operator).
+template <class S, class ST, bool PULL>
+__device__ __forceinline__ void outlet_neighbour(const KParams& p, const lbm_cell_class& K,
+                                                 int i0, int i1, int i2,
+                                                 typename ST::C (&g)[S::Q]) {
+  const int j0 = i0 - K.dir[0], j1 = i1 - K.dir[1], j2 = i2 - K.dir[2];
+  const int w0 = j0 < 0 ? j0 + p.n0 : (j0 >= p.n0 ? j0 - p.n0 : j0);
+  const int w1 = j1 < 0 ? j1 + p.n1 : (j1 >= p.n1 ? j1 - p.n1 : j1);
+  const int w2 = j2 < 0 ? j2 + p.n2 : (j2 >= p.n2 ? j2 - p.n2 : j2);
+  int c0[3], c1[3], c2[3];
+  axis_offsets(w0, p.n0, p.s0, c0);
+  axis_offsets(w1, p.n1, p.s1, c1);
+  axis_offsets(w2, p.n2, 1, c2);
+  const int nb = c0[1] + c1[1] + c2[1];
+  if (PULL) {
+    const lbm_cell_class& KN = p.ct[p.cls[nb]];
+    gather_pull<S, ST, true>(p, nb, c0, c1, c2, KN.no_stream, KN.flags, g);
+  } else {
+    gather_own<S, ST>(p, nb, g);
+  }
+}
+
 // The operator of a non-collision cell (one of LBM_OP_BOUNCE_BACK,
-// LBM_OP_EQUILIBRIUM, LBM_OP_OUTLET, LBM_OP_KEEP) on its populations f.
+// LBM_OP_EQUILIBRIUM, LBM_OP_OUTLET, LBM_OP_ANTI_BOUNCE_BACK, LBM_OP_KEEP)
+// on its populations f.
 template <class S, class ST, bool PULL>
 __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op, uint32_t flags,
                                                   int cell, int i0, int i1, int i2,
@@ -177,28 +208,36 @@ __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op,
     }
     equilibrium<S, C>(rho, u, f);
   } else if (op == LBM_OP_OUTLET) {
-    // f <- feq(rho_out, u(x - dir)) (equilibrium_outlet_p.py:63-73); the
-    // neighbour's velocity is taken from its streamed populations.
+    // f <- feq(rho_out, u(x - dir)) (equilibrium_outlet_p.py:63-73)
     const lbm_cell_class& K = p.ct[p.cls[cell]];
-    const int j0 = i0 - K.dir[0], j1 = i1 - K.dir[1], j2 = i2 - K.dir[2];
-    const int w0 = j0 < 0 ? j0 + p.n0 : (j0 >= p.n0 ? j0 - p.n0 : j0);
-    const int w1 = j1 < 0 ? j1 + p.n1 : (j1 >= p.n1 ? j1 - p.n1 : j1);
-    const int w2 = j2 < 0 ? j2 + p.n2 : (j2 >= p.n2 ? j2 - p.n2 : j2);
-    int c0[3], c1[3], c2[3];
-    axis_offsets(w0, p.n0, p.s0, c0);
-    axis_offsets(w1, p.n1, p.s1, c1);
-    axis_offsets(w2, p.n2, 1, c2);
-    const int nb = c0[1] + c1[1] + c2[1];
     C g[S::Q];
-    if (PULL) {
-      const lbm_cell_class& KN = p.ct[p.cls[nb]];
-      gather_pull<S, ST, true>(p, nb, c0, c1, c2, KN.no_stream, KN.flags, g);
-    } else {
-      gather_own<S, ST>(p, nb, g);
-    }
+    outlet_neighbour<S, ST, PULL>(p, K, i0, i1, i2, g);
     C rho_n, u[3];
     moments<S, C>(g, rho_n, u);
     equilibrium<S, C>((C)K.rho, u, f);
+  } else if (op == LBM_OP_ANTI_BOUNCE_BACK) {
+    // anti-bounce-back outlet (anti_bounce_back_outlet.py:79-92): for the
+    // populations v leaving through the outlet (e_v . dir = 1),
+    // f_opp(v) <- -f_v + w_v rho (2 + (e_v.u_w)^2 / cs^4 - |u_w|^2 / cs^2)
+    // with u_w = u + (u - u(x - dir)) / 2 and rho, u of the cell itself
+    const lbm_cell_class& K = p.ct[p.cls[cell]];
+    C g[S::Q];
+    outlet_neighbour<S, ST, PULL>(p, K, i0, i1, i2, g);
+    C rho, u[3], rho_n, un[3];
+    moments<S, C>(f, rho, u);
+    moments<S, C>(g, rho_n, un);
+    C uw[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uw[d] = u[d] + C(0.5) * (u[d] - un[d]);
+    const C uwuw = uw[0] * uw[0] + uw[1] * uw[1] + uw[2] * uw[2];
+    const int d0 = K.dir[0], d1 = K.dir[1], d2 = K.dir[2];
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) {
+      if (go<S>(q, 0) * d0 + go<S>(q, 1) * d1 + go<S>(q, 2) * d2 == 1) {
+        const C eu = eu_of<S, C>(q, uw);
+        f[S::opp[q]] = -f[q] + C(S::w[q]) * rho * (C(2) + C(9) * eu * eu - C(3) * uwuw);
+      }
+    }
   }
   // LBM_OP_KEEP: unchanged
 }
@@ -509,6 +548,7 @@ KernelFn pick_coll(int coll, bool mask, int mode) {
     case LBM_COLL_NONE: return pick_mask<S, ST, LBM_COLL_NONE>(mask, mode);
     case LBM_COLL_BGK: return pick_mask<S, ST, LBM_COLL_BGK>(mask, mode);
     case LBM_COLL_TRT: return pick_mask<S, ST, LBM_COLL_TRT>(mask, mode);
+    case LBM_COLL_BGK_GUO: return pick_mask<S, ST, LBM_COLL_BGK_GUO>(mask, mode);
     case LBM_COLL_KBC:
       if constexpr (S::ID == LBM_D3Q19) return nullptr;  // lettuce: D2Q9/D3Q27 only
       else return pick_mask<S, ST, LBM_COLL_KBC>(mask, mode);

@@ -1,5 +1,6 @@
 """Operators, flows and reporters (lettuce.ext counterparts)."""
 from ._operators import *  # noqa: F401,F403
+from ._force import *  # noqa: F401,F403
 from ._learned import *  # noqa: F401,F403
 from ._training import *  # noqa: F401,F403
 from ._flows import *  # noqa: F401,F403

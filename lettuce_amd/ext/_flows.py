@@ -26,7 +26,8 @@ from ._operators import (AntiBounceBackOutlet, BounceBackBoundary,
                          EquilibriumBoundaryPU, EquilibriumOutletP,
                          HalfwayBounceBackBoundary, QuadraticEquilibrium)
 
-__all__ = ["ExtFlow", "TaylorGreenVortex", "DecayingTurbulence", "Obstacle"]
+__all__ = ["ExtFlow", "TaylorGreenVortex", "DecayingTurbulence", "Obstacle",
+           "PoiseuilleFlow2D"]
 
 
 class ExtFlow(Flow, ABC):
@@ -116,6 +117,68 @@ class TaylorGreenVortex(ExtFlow):
     @property
     def post_boundaries(self):
         return []
+
+
+class PoiseuilleFlow2D(ExtFlow):
+    """Channel driven by a uniform acceleration (0.001, 0) between fullway
+    bounce-back walls at y = 0 and y = -1 (lettuce/ext/_flows/poiseuille.py:
+    18-86); run it with BGKCollision(tau, force=Guo(flow, tau,
+    flow.units.convert_acceleration_to_lu(flow.acceleration)))."""
+
+    def __init__(self, context, resolution, reynolds_number, mach_number,
+                 stencil: Optional[Stencil] = None, equilibrium=None,
+                 initialize_with_zeros: bool = True):
+        self.stencil = D2Q9() if stencil is None else (stencil() if callable(stencil)
+                                                      else stencil)
+        self.initialize_with_zeros = initialize_with_zeros
+        ExtFlow.__init__(self, context, resolution, reynolds_number, mach_number,
+                         self.stencil, equilibrium)
+
+    def make_resolution(self, resolution, stencil=None) -> List[int]:
+        if isinstance(resolution, int):
+            return [resolution] * self.stencil.d
+        if len(resolution) != self.stencil.d:
+            raise ValueError(f"resolution needs {self.stencil.d} entries")
+        return list(resolution)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return UnitConversion(reynolds_number=reynolds_number, mach_number=mach_number,
+                              characteristic_length_lu=resolution[0] - 1,
+                              characteristic_length_pu=1, characteristic_velocity_pu=1)
+
+    @property
+    def grid(self):
+        xyz = tuple(torch.linspace(0, 1, steps=n, device=self.context.device,
+                                   dtype=self.context.dtype) for n in self.resolution)
+        return torch.meshgrid(*xyz, indexing="ij")
+
+    def analytic_solution(self, t=0):
+        """u_x = a / (2 rho nu) (y - h)(1 - h - y), h half a lattice spacing
+        (poiseuille.py:31-41)"""
+        h = 0.5 / self.resolution[0]
+        x, y = self.grid
+        nu = self.units.viscosity_pu
+        ux = self.acceleration[0] / (2 * 1 * nu) * ((y - h) * (1 - h - y))
+        uy = torch.zeros_like(ux)
+        p = y * 0 + self.units.convert_density_lu_to_pressure_pu(1)
+        return p[None], torch.stack([ux, uy])
+
+    def initial_pu(self):
+        if self.initialize_with_zeros:
+            z = torch.zeros(self.resolution, device=self.context.device,
+                            dtype=self.context.dtype)
+            return z[None], torch.stack([z, z])
+        return self.analytic_solution()
+
+    @property
+    def post_boundaries(self):
+        mask = torch.zeros(self.resolution, dtype=torch.bool)
+        mask[:, [0, -1]] = True
+        return [BounceBackBoundary(mask)]
+
+    @property
+    def acceleration(self):
+        return self.context.convert_to_tensor([0.001, 0.0])
 
 
 class DecayingTurbulence(ExtFlow):

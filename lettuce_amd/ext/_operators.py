@@ -11,7 +11,7 @@ same kernels.
 import ctypes
 import numbers
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence, Union
+from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 import torch
@@ -72,6 +72,8 @@ class NativeCollisionDesc:
     tau_minus: float = 0.0
     index: int = 0
     owner: object = None
+    force_tau: float = 0.0                  # BGK_GUO
+    accel: Tuple[float, ...] = (0.0, 0.0, 0.0)
 
     def bind(self, flow):
         if self.kind == N.COLL_KBC:
@@ -142,11 +144,15 @@ class BGKCollision(Collision):
         return type(self).__name__
 
     def native_available(self) -> bool:
-        return self.force is None
+        return self.force is None or self.force.native_available()
 
     def native_generator(self, index: int):
         if self.force is not None:
-            return None
+            if not self.force.native_available():
+                return None
+            spec = self.force.native_generator()
+            return NativeCollisionDesc(N.COLL_BGK_GUO, tau=float(self.tau), index=index,
+                                       force_tau=spec["tau"], accel=spec["acceleration"])
         return NativeCollisionDesc(N.COLL_BGK, tau=float(self.tau), index=index)
 
 
@@ -424,8 +430,11 @@ class HalfwayBounceBackBoundary(Boundary):
 
 class AntiBounceBackOutlet(Boundary):
     """lettuce's anti-bounce-back outlet
-    (lettuce/ext/_boundary/anti_bounce_back_outlet.py:13-109); it has no
-    native kernel here."""
+    (lettuce/ext/_boundary/anti_bounce_back_outlet.py:13-109): on the outlet
+    column, for the populations v leaving the domain,
+    f_opp(v) <- -f_v + w_v rho (2 + (e_v.u_w)^2 / cs^4 - |u_w|^2 / cs^2) with
+    u_w = u + (u - u_neighbour) / 2; the incoming populations opp(v) are not
+    streamed there.  As in lettuce, `collision` is accepted and unused."""
 
     def __init__(self, direction: List[int], flow, collision=None):
         self.direction = list(direction)
@@ -433,7 +442,8 @@ class AntiBounceBackOutlet(Boundary):
         self._opposite = list(flow.stencil.opposite)
 
     def __call__(self, flow):
-        raise NotImplementedError("AntiBounceBackOutlet has no native kernel")
+        raise NotImplementedError(
+            "AntiBounceBackOutlet acts inside the native step only")
 
     def make_no_streaming_mask(self, shape: List[int], context):
         m = context.zero_tensor(shape, dtype=torch.bool)
@@ -447,7 +457,14 @@ class AntiBounceBackOutlet(Boundary):
         return m
 
     def native_available(self) -> bool:
-        return False
+        return True
 
     def native_generator(self, index: int):
-        return None
+        return NativeBoundaryDesc(N.OP_ANTI_BOUNCE_BACK, index, owner=self)
+
+    def _op_spec(self, flow):
+        d = flow.stencil.d
+        axes = ([0] + self.direction) if d == 2 else list(self.direction)
+        column = self.make_no_collision_mask(list(flow.resolution), flow.context)
+        return OpSpec(op=N.OP_ANTI_BOUNCE_BACK, direction=tuple(axes),
+                      column_mask=column, lattice_direction=tuple(self.direction))

@@ -16,7 +16,7 @@ from .. import _ops
 from .._simulation import Reporter
 
 __all__ = ["Observable", "ObservableReporter", "MaximumVelocity",
-           "IncompressibleKineticEnergy", "Enstrophy", "Mass"]
+           "IncompressibleKineticEnergy", "Enstrophy", "Mass", "ErrorReporter"]
 
 
 class Observable(ABC):
@@ -123,3 +123,37 @@ class ObservableReporter(Reporter):
                 self.out.append(entry)
             else:
                 print(*entry, file=self.out)
+
+
+class ErrorReporter(Reporter):
+    """L2 errors of u and p against an analytic solution every `interval`
+    steps, each divided by resolution^(d/2)
+    (lettuce/ext/_reporter/error_reporter.py:9-42)."""
+
+    periodic_interval = True
+
+    def __init__(self, analytical_solution, interval=1, out=sys.stdout):
+        super().__init__(interval)
+        self.analytical_solution = analytical_solution
+        self.out = [] if out is None else out
+        if not isinstance(self.out, list):
+            print("#error_u         error_p", file=self.out)
+
+    def __call__(self, simulation):
+        flow = simulation.flow
+        i = flow.i
+        if i % self.interval != 0:
+            return
+        t = simulation.units.convert_time_to_pu(i)
+        pref, uref = self.analytical_solution(t=t)
+        pref = flow.context.convert_to_tensor(pref)
+        uref = flow.context.convert_to_tensor(uref)
+        p, u = flow.p_pu, flow.u_pu
+        d = flow.stencil.d
+        resolution = torch.pow(torch.prod(flow.context.convert_to_tensor(p.size())), 1 / d)
+        err_u = torch.norm(u - uref) / resolution ** (d / 2)
+        err_p = torch.norm(p - pref) / resolution ** (d / 2)
+        if isinstance(self.out, list):
+            self.out.append([err_u.item(), err_p.item()])
+        else:
+            print(err_u.item(), err_p.item(), file=self.out)

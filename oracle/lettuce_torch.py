@@ -26,6 +26,10 @@ Pinning (what checks each part; see tests/test_oracle.py):
                       wall velocity, exact mass conservation
   masks               golden ncm/nsm from lettuce's Simulation
   TGV / observables   analytic decay, golden vectors
+  Guo forcing         a = 0 is BGK; mass kept and momentum raised by exactly
+                      a per cell and step (closed form)
+  anti-bounce-back    a uniform equilibrium is a fixed point; the wall
+                      velocity extrapolation by hand
   learned collision   zero MLP output = BGK, rates 1/tau = BGK, forced
                       rates against the inference formula; training step's
                       autograd gradient against central differences
@@ -41,6 +45,7 @@ __all__ = [
     "QuadraticEquilibrium", "NoCollision", "BGKCollision", "TRTCollision",
     "KBCCollision", "BounceBackBoundary", "EquilibriumBoundaryPU",
     "EquilibriumOutletP", "AntiBounceBackOutlet", "HalfwayBounceBackBoundary", "Flow",
+    "Guo",
     "TaylorGreenVortex", "DecayingTurbulence", "ObstacleChannel",
     "Simulation", "StreamingStrategy", "torch_gradient",
     "incompressible_kinetic_energy", "enstrophy", "mass",
@@ -243,13 +248,41 @@ class NoCollision:  # no_collision.py:10-11
         return flow.f
 
 
-class BGKCollision:  # bgk_collision.py:17-22
-    def __init__(self, tau):
+class BGKCollision:  # bgk_collision.py:12-22
+    def __init__(self, tau, force=None):
         self.tau = tau
+        self.force = force
 
     def __call__(self, flow):
-        feq = flow.equilibrium(flow, u=flow.u())
-        return flow.f - 1.0 / self.tau * (flow.f - feq)
+        u_eq = 0 if self.force is None else self.force.u_eq(flow)
+        u = flow.u() + u_eq
+        feq = flow.equilibrium(flow, u=u)
+        si = self.force.source_term(u) if self.force is not None else 0
+        return flow.f - 1.0 / self.tau * (flow.f - feq) + si
+
+
+class Guo:  # ext/_force/guo.py:9-43
+    def __init__(self, flow, tau, acceleration):
+        self.flow = flow
+        self.tau = tau
+        self.acceleration = torch.as_tensor(acceleration, dtype=flow.context.dtype)
+
+    def source_term(self, u):  # :15-26
+        e = torch.tensor(self.flow.stencil.e, dtype=u.dtype)
+        w = torch.tensor(self.flow.stencil.w, dtype=u.dtype)
+        d = self.flow.stencil.d
+        cs = self.flow.stencil.cs
+        emu = e.reshape(list(e.shape) + [1] * d) - u
+        eu = torch.einsum("ib,b...->i...", e, u)
+        eeu = torch.einsum("ia,i...->ia...", e, eu)
+        emu_eeu = emu / cs ** 2 + eeu / cs ** 4
+        emu_eeu_f = torch.einsum("ia...,a->i...", emu_eeu, self.acceleration)
+        return (1 - 1 / (2 * self.tau)) * w.reshape([-1] + [1] * d) * emu_eeu_f
+
+    def u_eq(self, flow=None):  # :28-32
+        flow = self.flow if flow is None else flow
+        a = self.acceleration.reshape([-1] + [1] * flow.stencil.d)
+        return 0.5 * a / flow.rho()
 
 
 class TRTCollision:  # trt_collision.py:16-27

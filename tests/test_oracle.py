@@ -342,3 +342,63 @@ def test_halfway_bounce_back_wall_position_and_mass():
     amp = float(flow.u()[0, 0, 1:-1].mul(torch.sin(torch.pi * (y[1:-1] - 0.5) / H)).sum()
                 / torch.sin(torch.pi * (y[1:-1] - 0.5) / H).pow(2).sum())
     assert np.isclose(amp / U, np.exp(-nu * (np.pi / H) ** 2 * steps), rtol=3e-3)
+
+
+# ------------------------------------------------- anti-bounce-back outlet
+def test_anti_bounce_back_keeps_a_uniform_equilibrium():
+    """feq_v + feq_opp(v) = w rho (2 + 9 (e.u)^2 - 3 |u|^2), so a uniform
+    equilibrium column is a fixed point of the outlet"""
+    for s, res in ((O.D2Q9(), [6, 5]), (O.D3Q27(), [5, 4, 3])):
+        flow = _Grid(s, res)
+        d = s.d
+        u = torch.zeros([d] + res, dtype=F64)
+        u[0] = 0.04
+        u[1] = -0.01
+        rho = torch.full([1] + res, 1.02, dtype=F64)
+        flow.f = flow.equilibrium(flow, rho=rho, u=u)
+        f0 = flow.f.clone()
+        abb = O.AntiBounceBackOutlet([1] + [0] * (d - 1), flow)
+        out = abb(flow)
+        torch.testing.assert_close(out, f0, rtol=0, atol=1e-15)
+
+
+def test_anti_bounce_back_extrapolates_the_wall_velocity():
+    """a linear velocity profile along x: u_w = u(N-1) + (u(N-1) - u(N-2)) / 2
+    (hand-computed), and only the incoming populations of the outlet column
+    change"""
+    s = O.D2Q9()
+    flow = _Grid(s, [4, 3])
+    ux = torch.tensor([0.01, 0.02, 0.03, 0.04], dtype=F64)[:, None].expand(4, 3)
+    u = torch.stack([ux, torch.zeros(4, 3, dtype=F64)])
+    rho = torch.ones(1, 4, 3, dtype=F64)
+    flow.f = flow.equilibrium(flow, rho=rho, u=u)
+    f0 = flow.f.clone()
+    out = O.AntiBounceBackOutlet([1, 0], flow)(flow).clone()
+    uw = 0.04 + 0.5 * (0.04 - 0.03)
+    for v in (1, 5, 8):                      # e_x = +1
+        o = s.opposite[v]
+        ex, ey = s.e[v]
+        expect = -f0[v, -1] + s.w[v] * 1.0 * (2 + 9 * (ex * uw) ** 2 - 3 * uw ** 2)
+        torch.testing.assert_close(out[o, -1], expect, rtol=0, atol=1e-15)
+    changed = (out != f0).any(dim=(1, 2))
+    assert sorted(torch.nonzero(changed).flatten().tolist()) == [3, 6, 7]
+    assert torch.equal(out[:, :-1], f0[:, :-1])
+
+
+# ------------------------------------------------------------- Guo forcing
+@pytest.mark.parametrize("s", [O.D2Q9(), O.D3Q19(), O.D3Q27()], ids=["D2Q9", "D3Q19", "D3Q27"])
+def test_guo_forcing_adds_exactly_the_acceleration(s):
+    """With u* = u + a/(2 rho): sum_q si = 0 and sum_q e si = (1 - 1/(2 tau)) a,
+    so BGK + Guo keeps the mass and raises the momentum of every cell by a
+    (j' = j - (j - rho u*)/tau + (1 - 1/(2 tau)) a = j + a)."""
+    flow = _perturbed(s, 4)
+    tau = 0.62
+    a = [0.002, -0.001, 0.0005][:s.d]
+    coll = O.BGKCollision(tau, force=O.Guo(flow, tau, a))
+    post = coll(flow)
+    torch.testing.assert_close(flow.rho(post), flow.rho(), rtol=0, atol=1e-14)
+    expect = flow.j() + torch.tensor(a, dtype=F64).reshape([-1] + [1] * s.d)
+    torch.testing.assert_close(flow.j(post), expect, rtol=0, atol=1e-14)
+    # a = 0 is plain BGK
+    zero = O.BGKCollision(tau, force=O.Guo(flow, tau, [0.0] * s.d))(flow)
+    torch.testing.assert_close(zero, O.BGKCollision(tau)(flow), rtol=0, atol=1e-15)
