@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "lbm_step_impl.h"
+#include "lbm_tb2.h"
 
 namespace lbm {
 KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode);
@@ -150,6 +151,17 @@ static int block_order(const KParams& p, int64_t nb) {
   // measured on MI355X (D3Q19 256^3 FP32): storage order 5.69 TB/s, XCD
   // chunks 5.57 TB/s, XCD slabs 4.76 TB/s
   return ORDER_LINEAR;
+}
+
+// the two-step kernel for this step, if one applies: periodic 3D lattice
+// without boundaries, pull streaming, z a multiple of 64 and y of TB_Y,
+// each population one contiguous block (LETTUCE_AMD_TB2=0 turns it off)
+static TbFn two_step_kernel(const lbm_step_args* a) {
+  if (env_int("LETTUCE_AMD_TB2", 0) == 0) return nullptr;
+  if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
+  if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
+  if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
+  return pick_tb2_kernel(a->stencil, a->storage, a->collision);
 }
 
 static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
@@ -304,9 +316,31 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
     return check_launch("lbm_run (learned)");
   }
   const void* bufs[2] = {a->f_in, a->f_out};
-  for (int64_t k = 0; k < nsteps; ++k) {
-    p.fin = bufs[k & 1];
-    p.fout = (void*)bufs[(k + 1) & 1];
+  int64_t launch = 0;  // launches so far; launch k reads bufs[k & 1]
+  // pairs of steps fused into one pass (two-step kernel); after nsteps the
+  // result must sit in bufs[nsteps & 1] as with single steps, so an odd
+  // number of pairs runs one pair as two single steps
+  TbFn tb = two_step_kernel(a);
+  int64_t pairs = tb ? nsteps / 2 : 0;
+  if (pairs % 2 != 0) --pairs;
+  if (pairs > 0) {
+    TbParams tp;
+    tp.k = p;
+    tp.cx = env_int("LETTUCE_AMD_TB2_PLANES", 16);
+    if (tp.cx < 1) tp.cx = 1;
+    tp.nzt = (int)(a->n[2] / 64);
+    tp.nyt = (int)(a->n[1] / TB_Y);
+    const int64_t nxt = (a->n[0] + tp.cx - 1) / tp.cx;
+    const int64_t nb = (int64_t)tp.nzt * tp.nyt * nxt;
+    for (int64_t k = 0; k < pairs; ++k, ++launch) {
+      tp.k.fin = bufs[launch & 1];
+      tp.k.fout = (void*)bufs[(launch + 1) & 1];
+      hipLaunchKernelGGL(tb, dim3((unsigned)nb), dim3(TB_THREADS), 0, (hipStream_t)stream, tp);
+    }
+  }
+  for (int64_t k = 2 * pairs; k < nsteps; ++k, ++launch) {
+    p.fin = bufs[launch & 1];
+    p.fout = (void*)bufs[(launch + 1) & 1];
     hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
   }
   return check_launch("lbm_run");

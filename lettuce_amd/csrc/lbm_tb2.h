@@ -1,0 +1,23 @@
+// Shared declarations of the two-step (temporally blocked) kernel.
+#pragma once
+
+#include "lbm_step_impl.h"
+
+namespace lbm {
+
+constexpr int TB_Y = 8;                        // interior rows per block
+constexpr int TB_ROWS = TB_Y + 2;              // with one halo row per side
+constexpr int TB_THREADS = 64 * (TB_ROWS + 1); // row wavefronts + z-edge wavefront
+
+struct TbParams {
+  KParams k;
+  int cx;   // planes per block along x
+  int nzt;  // tiles along z (n2 / 64)
+  int nyt;  // tiles along y (n1 / TB_Y)
+};
+using TbFn = void (*)(const TbParams);
+
+// two-step kernel for (stencil, storage, collision), or nullptr
+TbFn pick_tb2_kernel(int stencil, int storage, int coll);
+
+}  // namespace lbm

@@ -339,7 +339,7 @@ class EquilibriumOutletP(Boundary):
     leaving through the outlet are not streamed there."""
 
     def __init__(self, direction: List[int], flow, rho_outlet: float = 1.0):
-        self.direction = list(direction)
+        self.direction = [int(round(float(c))) for c in direction]
         self.context = flow.context
         self.rho_outlet = float(rho_outlet)
         self.velocities, self.index = _outlet_geometry(self.direction, flow)
@@ -437,7 +437,7 @@ class AntiBounceBackOutlet(Boundary):
     streamed there.  As in lettuce, `collision` is accepted and unused."""
 
     def __init__(self, direction: List[int], flow, collision=None):
-        self.direction = list(direction)
+        self.direction = [int(round(float(c))) for c in direction]
         self.velocities, self.index = _outlet_geometry(self.direction, flow)
         self._opposite = list(flow.stencil.opposite)
 
