@@ -7,7 +7,7 @@ namespace lbm {
 
 constexpr int TB_Y = 8;                        // interior rows per block
 constexpr int TB_ROWS = TB_Y + 2;              // with one halo row per side
-constexpr int TB_THREADS = 64 * (TB_ROWS + 1); // row wavefronts + z-edge wavefront
+constexpr int TB_THREADS = 64 * TB_ROWS;       // one wavefront per row
 
 struct TbParams {
   KParams k;

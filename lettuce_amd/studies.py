@@ -291,7 +291,10 @@ def train2d(iters=300, n=256, fields=32, holdout=8, steps=16, every=4, u_rms=0.0
 
     # rate_hi above 1/tau: a zero MLP output is then BGK exactly, and the
     # clamp does not block the gradient of rates near 1/tau
-    base = lt.LearnedParameters.seeded(seed, out_scale=0.1, rate_hi=rate_hi)
+    # the training step has no fallback: the inference kernel runs with an
+    # unbounded stable range so that it applies the same formula
+    base = lt.LearnedParameters.seeded(seed, out_scale=0.1, rate_hi=rate_hi,
+                                       stable_lo=-1e30, stable_hi=1e30)
     cfg = lt.TrainConfig.of(base, tau)
     with torch.no_grad():
         bgk_tr = err(lt.velocity_d2q9(lt.learned_rollout(f_tr, zero, cfg, steps, every))[1], ut_tr)
