@@ -92,6 +92,7 @@ _SIGNATURES = {
     "lbm_reduce_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _vp, _vp,
                         _vp], ctypes.c_int),
     "lbm_enstrophy_f64": ([_i32, _I64_3, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_run_plan": ([ctypes.POINTER(LbmStepArgs), _i64, _vp], ctypes.c_int),
     "lbm_scan_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _vp, _vp], ctypes.c_int),
     "lbm_link_force_f64": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _i64,
                             _vp, _vp, _vp], ctypes.c_int),

@@ -149,6 +149,14 @@ class NativeStep:
         if nsteps % 2 == 1:
             self.flow.swap()
 
+    def plan(self, nsteps: int):
+        """(two-step launches, single-step launches) that run(nsteps) makes"""
+        self._bind_buffers()
+        out = (ctypes.c_int64 * 2)()
+        N.check(N.lib().lbm_run_plan(ctypes.byref(self.args), int(nsteps), out),
+                "lbm_run_plan")
+        return int(out[0]), int(out[1])
+
 
 class Simulation:
     def __init__(self, flow, collision, reporter: List["Reporter"],

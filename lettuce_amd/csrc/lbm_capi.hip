@@ -170,6 +170,15 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   return pick_tb2_kernel(a->stencil, a->storage, a->collision);
 }
 
+// pairs of steps fused into one pass by the two-step kernel.  After nsteps
+// the result must sit in bufs[nsteps & 1] as with single steps, so an odd
+// number of pairs runs one pair as two single steps.
+static int64_t two_step_pairs(TbFn tb, int64_t nsteps) {
+  int64_t pairs = tb ? nsteps / 2 : 0;
+  if (pairs % 2 != 0) --pairs;
+  return pairs;
+}
+
 static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
   if (!a) return fail(LBM_E_ARG, "NULL args");
   if (a->abi_version != LBM_ABI_VERSION)
@@ -306,6 +315,20 @@ int lbm_step(const lbm_step_args* a, void* stream) {
   return lbm_run(a, 1, stream);
 }
 
+int lbm_run_plan(const lbm_step_args* a, int64_t nsteps, int64_t* plan) {
+  if (nsteps < 0) return fail(LBM_E_ARG, "negative step count");
+  if (!plan) return fail(LBM_E_ARG, "NULL plan");
+  KParams p;
+  KernelFn fn;
+  dim3 grid, block;
+  int rc = prepare_step(a, p, fn, grid, block);
+  if (rc) return rc;
+  const int64_t pairs = a->collision == LBM_COLL_LEARNED ? 0 : two_step_pairs(two_step_kernel(a), nsteps);
+  plan[0] = pairs;
+  plan[1] = nsteps - 2 * pairs;
+  return LBM_OK;
+}
+
 int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   if (nsteps < 0) return fail(LBM_E_ARG, "negative step count");
   KParams p;
@@ -323,12 +346,8 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   }
   const void* bufs[2] = {a->f_in, a->f_out};
   int64_t launch = 0;  // launches so far; launch k reads bufs[k & 1]
-  // pairs of steps fused into one pass (two-step kernel); after nsteps the
-  // result must sit in bufs[nsteps & 1] as with single steps, so an odd
-  // number of pairs runs one pair as two single steps
   TbFn tb = two_step_kernel(a);
-  int64_t pairs = tb ? nsteps / 2 : 0;
-  if (pairs % 2 != 0) --pairs;
+  const int64_t pairs = two_step_pairs(tb, nsteps);
   if (pairs > 0) {
     TbParams tp;
     tp.k = p;
@@ -337,7 +356,10 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
     tp.nzt = (int)(a->n[2] / 64);
     tp.nyt = (int)(a->n[1] / TB_Y);
     const int64_t nxt = (a->n[0] + tp.cx - 1) / tp.cx;
-    const int64_t nb = (int64_t)tp.nzt * tp.nyt * nxt;
+    const int64_t ntiles = (int64_t)tp.nzt * tp.nyt * nxt;
+    tp.ntiles = (int)ntiles;
+    tp.per_xcd = env_int("LETTUCE_AMD_TB2_XCD", 1) ? (int)((ntiles + 7) / 8) : 0;
+    const int64_t nb = tp.per_xcd ? 8 * (int64_t)tp.per_xcd : ntiles;
     for (int64_t k = 0; k < pairs; ++k, ++launch) {
       tp.k.fin = bufs[launch & 1];
       tp.k.fout = (void*)bufs[(launch + 1) & 1];

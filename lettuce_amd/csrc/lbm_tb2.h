@@ -7,13 +7,15 @@ namespace lbm {
 
 constexpr int TB_Y = 8;                        // interior rows per block
 constexpr int TB_ROWS = TB_Y + 2;              // with one halo row per side
-constexpr int TB_THREADS = 64 * TB_ROWS;       // one wavefront per row
+constexpr int TB_THREADS = 64 * (TB_ROWS + 1); // row wavefronts + z-edge wavefront
 
 struct TbParams {
   KParams k;
-  int cx;   // planes per block along x
-  int nzt;  // tiles along z (n2 / 64)
-  int nyt;  // tiles along y (n1 / TB_Y)
+  int cx;       // planes per block along x
+  int nzt;      // tiles along z (n2 / 64)
+  int nyt;      // tiles along y (n1 / TB_Y)
+  int ntiles;   // nzt * nyt * x chunks
+  int per_xcd;  // 0: block b takes tile b; else tiles per XCD (see tb2_kernel)
 };
 using TbFn = void (*)(const TbParams);
 

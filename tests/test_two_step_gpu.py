@@ -46,10 +46,12 @@ class _Flow(lt.ExtFlow):
         self.f = torch.from_numpy(self._f0)
 
 
-def _run(f0, storage, layout, coll, steps):
+def _run(f0, storage, layout, coll, steps, pairs=None):
     c = lt.Context("cuda:0", dtype=F32, storage=storage, layout=layout)
     flow = _Flow(c, f0.astype(np.float32))
     sim = lt.Simulation(flow, coll, [], lt.StreamingStrategy.PRE_STREAMING)
+    if pairs is not None:
+        assert sim._collide_and_stream.plan(steps) == (pairs, steps - 2 * pairs)
     sim(steps)
     torch.cuda.synchronize()
     return (flow._state.detach().cpu().contiguous().view(-1).view(torch.uint8).numpy(),
@@ -74,10 +76,11 @@ def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, 
             "none": lambda: lt.NoCollision()}[coll]
     for steps in (2, 5, 6, 8):
         monkeypatch.setenv("LETTUCE_AMD_TB2", "0")
-        ref_bits, ref = _run(f0, storage, layout, make(), steps)
+        ref_bits, ref = _run(f0, storage, layout, make(), steps, pairs=0)
         monkeypatch.setenv("LETTUCE_AMD_TB2", "2")
         monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", str(planes))
-        got_bits, got = _run(f0, storage, layout, make(), steps)
+        pairs = steps // 2 - (steps // 2) % 2
+        got_bits, got = _run(f0, storage, layout, make(), steps, pairs=pairs)
         if coll == "none":
             assert np.array_equal(got_bits, ref_bits), (steps, int((got_bits != ref_bits).sum()))
         else:
@@ -93,8 +96,28 @@ def test_two_step_kernel_matches_the_oracle(layout, monkeypatch):
     f0 = _f0(res, seed=2)
     monkeypatch.setenv("LETTUCE_AMD_TB2", "1")
     monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", "3")
-    _, got = _run(f0, "f32", layout, lt.BGKCollision(0.58), 40)
+    _, got = _run(f0, "f32", layout, lt.BGKCollision(0.58), 40, pairs=20)
     oflow = _Grid(O.D3Q19(), res, F32, f0.astype(np.float32))
     O.Simulation(oflow, O.BGKCollision(0.58), [], O.StreamingStrategy.PRE_STREAMING)(40)
     ref = oflow.f.double().numpy()
     assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5
+
+
+def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
+    """FP32 only by default; no two-step launch with boundaries, push
+    streaming, or extents that do not tile"""
+    monkeypatch.delenv("LETTUCE_AMD_TB2", raising=False)
+    f0 = _f0([4, 16, 64])
+    for storage, pairs in (("f32", 4), ("f16_shifted", 0)):
+        c = lt.Context("cuda:0", dtype=F32, storage=storage)
+        sim = lt.Simulation(_Flow(c, f0.astype(np.float32)), lt.BGKCollision(0.6), [],
+                            lt.StreamingStrategy.PRE_STREAMING)
+        assert sim._collide_and_stream.plan(9) == (pairs, 9 - 2 * pairs)
+    c = lt.Context("cuda:0", dtype=F32)
+    sim = lt.Simulation(_Flow(c, f0.astype(np.float32)), lt.BGKCollision(0.6), [],
+                        lt.StreamingStrategy.POST_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (0, 8)
+    f1 = _f0([4, 12, 64])
+    sim = lt.Simulation(_Flow(c, f1.astype(np.float32)), lt.BGKCollision(0.6), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (0, 8)
