@@ -163,12 +163,15 @@ def copy_ceiling(nbytes, steps, warmup):
     return out
 
 
-def load_traffic(name):
+def load_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` on workload `name` from
+    profiles/traffic_<name>.json (None when no profile of that kernel)"""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch") if d.get("launch") == kernel else None
 
 
 def main():
@@ -219,7 +222,7 @@ def main():
     launches = pairs + singles
     launch_s = gpu_s / launches
     achieved = (2 * pairs + singles) * bytes_per_cell * cells / launches / launch_s / 1e9
-    traffic = load_traffic(args.workload)
+    traffic = load_traffic(args.workload, "two-step" if pairs else "single-step")
     line = {
         "metric": "MLUPS",
         "value": round(mlups, 1),
@@ -250,6 +253,11 @@ def main():
                      "launch_us": round(launch_s * 1e6, 2)},
         "wall_s": round(wall_s, 4),
     }
+    if traffic:
+        # what the HBM actually moved per launch (rocprofv3 FETCH_SIZE and
+        # WRITE_SIZE, calibrated on a copy), at the launch time measured here
+        line["roofline"]["hbm_achieved"] = round(traffic / launch_s / 1e9, 1)
+        line["roofline"]["hbm_frac"] = round(traffic / launch_s / 1e9 / PEAK_HBM_GBS, 4)
     ceiling = copy_ceiling(S.q * cells * BYTES[storage], 20, 3)
     line["roofline"]["measured_copy_ceiling"] = ceiling["best_GBps"]
     line["roofline"]["frac_of_copy_ceiling"] = round(achieved / ceiling["best_GBps"], 4)

@@ -153,13 +153,14 @@ static int block_order(const KParams& p, int64_t nb) {
   return ORDER_LINEAR;
 }
 
-// the two-step kernel for this step, if one applies: periodic D3Q19 without
-// boundaries, pull streaming, z a multiple of 64 and y of TB_Y.  Used by
-// default for FP32 storage, where the single step is bound by HBM; the
-// 16-bit formats stream half the bytes and run faster one step at a time
-// (measured on MI355X, D3Q19 BGK 256^3: FP32 38.8k -> 54.4k MLUPS, FP16
-// 78.5k -> 65.8k).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it for every
-// storage format it supports.
+// the two-step kernel for this step, if one applies: periodic D3Q19 or
+// D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
+// TB_Y.  Used by default for FP32 storage, where the single step is bound by
+// HBM (measured on MI355X, D3Q19 BGK 256^3: 38.8k -> 58.4k MLUPS); the
+// 16-bit formats stream half the bytes and gain nothing (FP16 79.0k ->
+// 79.6k).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it for every storage
+// format it supports; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64)
+// and LETTUCE_AMD_TB2_XCD=0 the plain tile order.
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
@@ -351,7 +352,7 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   if (pairs > 0) {
     TbParams tp;
     tp.k = p;
-    tp.cx = env_int("LETTUCE_AMD_TB2_PLANES", 32);
+    tp.cx = env_int("LETTUCE_AMD_TB2_PLANES", 64);
     if (tp.cx < 1) tp.cx = 1;
     tp.nzt = (int)(a->n[2] / 64);
     tp.nyt = (int)(a->n[1] / TB_Y);

@@ -174,6 +174,9 @@ TbFn pick_tb2_coll(int coll) {
     case LBM_COLL_NONE: return tb2_kernel<S, ST, LBM_COLL_NONE>;
     case LBM_COLL_BGK: return tb2_kernel<S, ST, LBM_COLL_BGK>;
     case LBM_COLL_TRT: return tb2_kernel<S, ST, LBM_COLL_TRT>;
+    case LBM_COLL_KBC:
+      if constexpr (S::ID == LBM_D3Q27) return tb2_kernel<S, ST, LBM_COLL_KBC>;
+      else return nullptr;
   }
   return nullptr;
 }
@@ -190,6 +193,7 @@ TbFn pick_tb2(int storage, int coll) {
 
 TbFn pick_tb2_kernel(int stencil, int storage, int coll) {
   if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll);
+  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll);
   return nullptr;
 }
 
