@@ -155,16 +155,17 @@ static int block_order(const KParams& p, int64_t nb) {
 
 // the two-step kernel for this step, if one applies: periodic D3Q19 or
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
-// TB_Y.  Used by default for FP32 storage, where the single step is bound by
-// HBM (measured on MI355X, D3Q19 BGK 256^3: 38.8k -> 58.4k MLUPS); the
-// 16-bit formats stream half the bytes and gain nothing (FP16 79.0k ->
-// 79.6k).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it for every storage
-// format it supports; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64)
-// and LETTUCE_AMD_TB2_XCD=0 the plain tile order.
+// TB_Y.  Used by default for D3Q19 with FP32 storage, where the single step
+// is bound by HBM (measured on MI355X, BGK 256^3: 38.8k -> 58.0k MLUPS).
+// The 16-bit formats stream half the bytes and gain nothing (FP16 79.0k ->
+// 79.6k); D3Q27 KBC loses (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave
+// one block per CU).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it
+// applies; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64) and
+// LETTUCE_AMD_TB2_XCD=0 the plain tile order.
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
-  if (mode == 1 && a->storage != LBM_F32) return nullptr;
+  if (mode == 1 && (a->storage != LBM_F32 || a->stencil != LBM_D3Q19)) return nullptr;
   if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
