@@ -32,6 +32,9 @@ WORKLOADS = {
     "tgv3d_d3q19_bgk_256_f16s": ("D3Q19", "bgk", "f16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_bgk_256_bf16s": ("D3Q19", "bgk", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q27_kbc_256_f32": ("D3Q27", "kbc", "f32", 256, 1600, 0.1),
+    "tgv3d_d3q27_kbc_256_bf16s": ("D3Q27", "kbc", "bf16_shifted", 256, 1600, 0.1),
+    "tgv3d_d3q27_learned_256_f32": ("D3Q27", "learned", "f32", 256, 1600, 0.1),
+    "tgv3d_d3q27_learned_256_bf16s": ("D3Q27", "learned", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_none_256_f32": ("D3Q19", "none", "f32", 256, 1600, 0.1),
     "tgv2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.1),
     # channel + cylinder (D = 64, halfway bounce-back, equilibrium in/outlet)

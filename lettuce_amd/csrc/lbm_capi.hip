@@ -17,8 +17,8 @@ namespace lbm {
 KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q19(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
-int launch_learned(const KParams& p, const lbm_learned* l, int storage, bool mask, int mode,
-                   hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out);
+int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int storage, bool mask,
+                   int mode, hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out);
 KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q19(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
@@ -212,7 +212,8 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
 
   fn = nullptr;
   if (a->collision == LBM_COLL_LEARNED) {
-    if (a->stencil != LBM_D2Q9) return fail(LBM_E_UNSUPPORTED, "the learned collision is D2Q9 only");
+    if (a->stencil != LBM_D2Q9 && a->stencil != LBM_D3Q27)
+      return fail(LBM_E_UNSUPPORTED, "the learned collision is defined for D2Q9 and D3Q27");
     if (a->storage == LBM_F64)
       return fail(LBM_E_UNSUPPORTED, "the learned collision computes in FP32; use F32 or shifted storage");
     if (a->n[2] % 64 != 0)
@@ -341,7 +342,7 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   if (nsteps == 0) return LBM_OK;
   g_err[0] = 0;
   if (a->collision == LBM_COLL_LEARNED) {
-    rc = launch_learned(p, a->learned, a->storage, a->cell_class != nullptr, a->streaming,
+    rc = launch_learned(p, a->learned, a->stencil, a->storage, a->cell_class != nullptr, a->streaming,
                         (hipStream_t)stream, nsteps, a->f_in, a->f_out);
     if (rc) return fail(rc, "no learned kernel for storage %d", a->storage);
     return check_launch("lbm_run (learned)");

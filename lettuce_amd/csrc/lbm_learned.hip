@@ -1,10 +1,17 @@
-// Learned collision for D2Q9, fused into the stream + collide step.
+// Learned collision for D2Q9 and D3Q27, fused into the stream + collide step.
 //
 // Per cell: KBC split of df = f - feq into the shear part ds and the
-// higher-order part dh (raw moments xxy, xyy, xxyy only); a 6 -> 32 -> 32 -> 3
-// SiLU MLP maps the six non-equilibrium moments to the relaxation rates of
-// xxy, xyy, xxyy:
-//   f' = f - ds / tau - sum_g r_g m_g(dh) phi_g.
+// higher-order part dh; a 6 -> 32 -> 32 -> 3 SiLU MLP maps six
+// non-equilibrium moments to three relaxation rates r_g of groups of
+// higher-order raw moments of dh:
+//   f' = f - ds / tau - sum_g r_g sum_{k in g} m_k(dh) phi_k,
+// with phi_k the dual vector of raw moment m_k.
+//   D2Q9   inputs T, N, Pxy, xxy, xyy, xxyy of df; groups {xxy}, {xyy},
+//          {xxyy} (dh holds only these three moments).
+//   D3Q27  inputs T, Nxz, Nyz, Pxy, Pxz, Pyz of df (the second-order
+//          moments KBC's shear part carries); groups: the raw moments
+//          x^a y^b z^c (a, b, c <= 2) of order a + b + c = 3, of order 4,
+//          and of order 5 and 6 (dh holds no moment of order <= 2).
 // Cells whose raw rates leave [stable_lo, stable_hi] (or are NaN) take the
 // KBC update (lettuce/ext/_collision/kbc_collision.py:96-160).
 //
@@ -86,10 +93,144 @@ __device__ constexpr float PHI[3][9] = {
     {0.f, -0.5f, 0.f, 0.5f, 0.f, 0.25f, -0.25f, -0.25f, 0.25f},
     {1.f, -0.5f, -0.5f, -0.5f, -0.5f, 0.25f, 0.25f, 0.25f, 0.25f}};
 
-template <class ST, bool MASK, int MODE>
+// lattice-specific parts: the six MLP inputs and the rate-weighted update
+template <class S>
+struct LearnedModel;
+
+template <>
+struct LearnedModel<D2Q9> {
+  __device__ static void features(const float (&df)[9], float sc, float (&x)[6]) {
+    const float xx = df[1] + df[3] + df[5] + df[6] + df[7] + df[8];
+    const float yy = df[2] + df[4] + df[5] + df[6] + df[7] + df[8];
+    const float xy = df[5] - df[6] + df[7] - df[8];
+    x[0] = (xx + yy) * sc;
+    x[1] = (xx - yy) * sc;
+    x[2] = xy * sc;
+    x[3] = (df[5] + df[6] - df[7] - df[8]) * sc;
+    x[4] = (df[5] - df[6] - df[7] + df[8]) * sc;
+    x[5] = (df[5] + df[6] + df[7] + df[8]) * sc;
+  }
+  __device__ static void update(float (&f)[9], const float (&ds)[9], const float (&dh)[9],
+                                const float (&rate)[3], float tau_inv) {
+    const float mg[3] = {dh[5] + dh[6] - dh[7] - dh[8], dh[5] - dh[6] - dh[7] + dh[8],
+                         dh[5] + dh[6] + dh[7] + dh[8]};
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      float upd = rate[0] * mg[0] * PHI[0][q];
+      upd += rate[1] * mg[1] * PHI[1][q];
+      upd += rate[2] * mg[2] * PHI[2][q];
+      f[q] = f[q] - ds[q] * tau_inv - upd;
+    }
+  }
+};
+
+// D3Q27: the raw moments x^a y^b z^c (a, b, c in 0..2) are a tensor product
+// of the 1D moments (1, x, x^2) at x in {-1, 0, 1}, so moments and dual
+// vectors are three 1D transforms:
+//   to moments   (v-, v0, v+) -> (v- + v0 + v+, v+ - v-, v+ + v-)
+//   back         (m0, m1, m2) -> ((m2 - m1) / 2, m0 - m2, (m1 + m2) / 2)
+template <>
+struct LearnedModel<D3Q27> {
+  using S = D3Q27;
+  __device__ static void features(const float (&df)[27], float sc, float (&x)[6]) {
+    float xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+#pragma unroll
+    for (int q = 1; q < 27; ++q) {
+      const int ex = S::e[q][0], ey = S::e[q][1], ez = S::e[q][2];
+      if (ex != 0) xx += df[q];
+      if (ey != 0) yy += df[q];
+      if (ez != 0) zz += df[q];
+      if (ex * ey != 0) xy += (ex * ey > 0) ? df[q] : -df[q];
+      if (ex * ez != 0) xz += (ex * ez > 0) ? df[q] : -df[q];
+      if (ey * ez != 0) yz += (ey * ez > 0) ? df[q] : -df[q];
+    }
+    x[0] = (xx + yy + zz) * sc;
+    x[1] = (xx - zz) * sc;
+    x[2] = (yy - zz) * sc;
+    x[3] = xy * sc;
+    x[4] = xz * sc;
+    x[5] = yz * sc;
+  }
+  __device__ static void update(float (&f)[27], const float (&ds)[27], const float (&dh)[27],
+                                const float (&rate)[3], float tau_inv) {
+    // t[i][j][k]: i, j, k = velocity component + 1 along x, y, z
+    float t[3][3][3];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1] = dh[q];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float a = t[0][j][k], b = t[1][j][k], c = t[2][j][k];
+        t[0][j][k] = a + b + c;
+        t[1][j][k] = c - a;
+        t[2][j][k] = c + a;
+      }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float a = t[i][0][k], b = t[i][1][k], c = t[i][2][k];
+        t[i][0][k] = a + b + c;
+        t[i][1][k] = c - a;
+        t[i][2][k] = c + a;
+      }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float a = t[i][j][0], b = t[i][j][1], c = t[i][j][2];
+        t[i][j][0] = a + b + c;
+        t[i][j][1] = c - a;
+        t[i][j][2] = c + a;
+      }
+    // t[a][b][c] = m_abc(dh); weight by the rate of its order group
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int o = a + b + c;
+          t[a][b][c] *= o <= 2 ? 0.f : (o == 3 ? rate[0] : (o == 4 ? rate[1] : rate[2]));
+        }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float m0 = t[0][j][k], m1 = t[1][j][k], m2 = t[2][j][k];
+        t[0][j][k] = 0.5f * (m2 - m1);
+        t[1][j][k] = m0 - m2;
+        t[2][j][k] = 0.5f * (m1 + m2);
+      }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float m0 = t[i][0][k], m1 = t[i][1][k], m2 = t[i][2][k];
+        t[i][0][k] = 0.5f * (m2 - m1);
+        t[i][1][k] = m0 - m2;
+        t[i][2][k] = 0.5f * (m1 + m2);
+      }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float m0 = t[i][j][0], m1 = t[i][j][1], m2 = t[i][j][2];
+        t[i][j][0] = 0.5f * (m2 - m1);
+        t[i][j][1] = m0 - m2;
+        t[i][j][2] = 0.5f * (m1 + m2);
+      }
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+      f[q] = f[q] - ds[q] * tau_inv - t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];
+  }
+};
+
+template <class S, class ST, bool MASK, int MODE>
 __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const LearnedArgs la) {
-  using S = D2Q9;
   using C = float;
+  constexpr int Q = S::Q;
   constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
   constexpr bool PUSH = (MODE & LBM_POST_STREAMING) != 0;
   const int lane = threadIdx.x & 63;
@@ -121,27 +262,21 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
       flags = K.flags;
       op = K.op;
     }
-    C f[9];
+    C f[Q];
     if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
     else gather_own<S, ST>(p, cell, f);
 
     // non-equilibrium part and the MLP inputs
-    C rho, u[3], fe[9], df[9], ds[9], dh[9];
+    C rho, u[3], fe[Q], df[Q], ds[Q], dh[Q];
     moments<S, C>(f, rho, u);
     equilibrium<S, C>(rho, u, fe);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) df[q] = f[q] - fe[q];
+    for (int q = 0; q < Q; ++q) df[q] = f[q] - fe[q];
     kbc_shear<S, C>(df, ds);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) dh[q] = df[q] - ds[q];
-    const C xx = df[1] + df[3] + df[5] + df[6] + df[7] + df[8];
-    const C yy = df[2] + df[4] + df[5] + df[6] + df[7] + df[8];
-    const C xy = df[5] - df[6] + df[7] - df[8];
-    const C sc = la.in_scale / rho;
-    const C feat[6] = {(xx + yy) * sc, (xx - yy) * sc, xy * sc,
-                       (df[5] + df[6] - df[7] - df[8]) * sc,
-                       (df[5] - df[6] - df[7] + df[8]) * sc,
-                       (df[5] + df[6] + df[7] + df[8]) * sc};
+    for (int q = 0; q < Q; ++q) dh[q] = df[q] - ds[q];
+    C feat[6];
+    LearnedModel<S>::features(df, la.in_scale / rho, feat);
     bf16x8 x8 = zero8;
 #pragma unroll
     for (int k = 0; k < 6; ++k) x8[k] = (__bf16)feat[k];
@@ -190,15 +325,7 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
       if (fallback) {
         collide_kbc<S, C>(f, cp);
       } else {
-        const C mg[3] = {dh[5] + dh[6] - dh[7] - dh[8], dh[5] - dh[6] - dh[7] + dh[8],
-                         dh[5] + dh[6] + dh[7] + dh[8]};
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          C upd = rate[0] * mg[0] * PHI[0][q];
-          upd += rate[1] * mg[1] * PHI[1][q];
-          upd += rate[2] * mg[2] * PHI[2][q];
-          f[q] = f[q] - ds[q] * tau_inv - upd;
-        }
+        LearnedModel<S>::update(f, ds, dh, rate, tau_inv);
       }
     } else {
       apply_boundary_op<S, ST, PULL>(p, op, flags, cell, i0, i1, i2, f);
@@ -209,34 +336,41 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
 
 using LearnedFn = void (*)(const KParams, const LearnedArgs);
 
-template <class ST, bool MASK>
+template <class S, class ST, bool MASK>
 LearnedFn learned_mode(int mode) {
   switch (mode) {
-    case 0: return learned_kernel<ST, MASK, 0>;
-    case 1: return learned_kernel<ST, MASK, 1>;
-    case 2: return learned_kernel<ST, MASK, 2>;
-    case 3: return learned_kernel<ST, MASK, 3>;
+    case 0: return learned_kernel<S, ST, MASK, 0>;
+    case 1: return learned_kernel<S, ST, MASK, 1>;
+    case 2: return learned_kernel<S, ST, MASK, 2>;
+    case 3: return learned_kernel<S, ST, MASK, 3>;
   }
   return nullptr;
 }
 
-template <class ST>
+template <class S, class ST>
 LearnedFn learned_mask(bool mask, int mode) {
-  return mask ? learned_mode<ST, true>(mode) : learned_mode<ST, false>(mode);
+  return mask ? learned_mode<S, ST, true>(mode) : learned_mode<S, ST, false>(mode);
 }
 
-LearnedFn pick_learned_kernel(int storage, bool mask, int mode) {
+template <class S>
+LearnedFn learned_storage(int storage, bool mask, int mode) {
   switch (storage) {
-    case LBM_F32: return learned_mask<StF32>(mask, mode);
-    case LBM_F16_SHIFTED: return learned_mask<StF16>(mask, mode);
-    case LBM_BF16_SHIFTED: return learned_mask<StBF16>(mask, mode);
+    case LBM_F32: return learned_mask<S, StF32>(mask, mode);
+    case LBM_F16_SHIFTED: return learned_mask<S, StF16>(mask, mode);
+    case LBM_BF16_SHIFTED: return learned_mask<S, StBF16>(mask, mode);
   }
   return nullptr;  // FP64 storage: the learned collision computes in FP32
 }
 
-int launch_learned(const KParams& p, const lbm_learned* l, int storage, bool mask, int mode,
-                   hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out) {
-  LearnedFn fn = pick_learned_kernel(storage, mask, mode);
+LearnedFn pick_learned_kernel(int stencil, int storage, bool mask, int mode) {
+  if (stencil == LBM_D2Q9) return learned_storage<D2Q9>(storage, mask, mode);
+  if (stencil == LBM_D3Q27) return learned_storage<D3Q27>(storage, mask, mode);
+  return nullptr;
+}
+
+int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int storage, bool mask,
+                   int mode, hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out) {
+  LearnedFn fn = pick_learned_kernel(stencil, storage, mask, mode);
   if (!fn) return LBM_E_UNSUPPORTED;
   LearnedArgs la;
   la.packed = (const LearnedPacked*)l->packed;

@@ -15,7 +15,7 @@ import torch
 
 from .. import _native as N
 from .._simulation import Collision
-from .._stencil import D2Q9
+from .._stencil import D2Q9, D3Q27
 from ._operators import NativeCollisionDesc, apply_collision
 
 __all__ = ["LearnedParameters", "LearnedCollision"]
@@ -76,9 +76,10 @@ class LearnedParameters:
 
 
 class LearnedCollision(Collision):
-    """D2Q9 learned collision; cells whose predicted rates leave the stable
-    range relax with KBC.  tau is the flow's relaxation_parameter_lu, like
-    KBC."""
+    """D2Q9 / D3Q27 learned collision; cells whose predicted rates leave the
+    stable range relax with KBC.  tau is the flow's relaxation_parameter_lu,
+    like KBC.  D2Q9 rates act on the moments xxy, xyy, xxyy; D3Q27 rates on
+    the raw moments of order 3, of order 4 and of orders 5-6."""
 
     def __init__(self, params: Optional[LearnedParameters] = None):
         self.params = params or LearnedParameters.seeded(0)
@@ -95,8 +96,8 @@ class LearnedCollision(Collision):
         return NativeCollisionDesc(N.COLL_LEARNED, index=index, owner=self)
 
     def _bind(self, desc, flow):
-        if not isinstance(flow.stencil, D2Q9):
-            raise NotImplementedError("the learned collision is defined for D2Q9")
+        if not isinstance(flow.stencil, (D2Q9, D3Q27)):
+            raise NotImplementedError("the learned collision is defined for D2Q9 and D3Q27")
         desc.tau = flow.units.relaxation_parameter_lu
         self._packed = self.params.packed(flow.context.device)
         s = N.LbmLearned()

@@ -164,11 +164,14 @@ def cylinder(steps=200000, nx=2048, ny=512, diameter=64, every=50, storage="f32"
             "mlups_incl_forces": round(steps * nx * ny / 1e6 / wall, 1)}
 
 
-def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=0):
-    """D3Q27 decaying isotropic turbulence, KBC, E(k) ~ k^4 exp(-2 (k/k0)^2),
-    u_rms = 0.05 and nu = 1e-4 in lattice units; FP32 storage and shifted
-    BF16 / FP16 storage against FP64: kinetic energy, the dissipation rate
-    -dE/dt and nu <|curl u|^2>, every `every` steps."""
+def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=0,
+                 learned=True):
+    """D3Q27 decaying isotropic turbulence, E(k) ~ k^4 exp(-2 (k/k0)^2),
+    u_rms = 0.05 and nu = 1e-4 in lattice units: KBC with FP32 storage and
+    shifted BF16 / FP16 storage, and the learned collision (weights seeded
+    with `seed`, KBC fallback) with FP32 and shifted BF16 storage, each
+    against KBC in FP64: kinetic energy, the dissipation rate -dE/dt and the
+    enstrophy, every `every` steps."""
     out = {"study": "turbulence3d_d3q27_kbc", "grid": [res] * 3, "steps": steps,
            "every": every, "nu_lu": nu, "u_rms_lu": u_rms}
     # lettuce scales the field to max|u| = Ma cs (decayingturbulence.py:167-169):
@@ -185,14 +188,20 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
     re = res * u_max / nu
     out["ma"], out["re"] = ma, re
     series, timing = {}, {}
-    for name, dtype, storage in (("f64", torch.float64, "f64"), ("f32", torch.float32, "f32"),
-                                 ("bf16_shifted", torch.float32, "bf16_shifted"),
-                                 ("f16_shifted", torch.float32, "f16_shifted")):
+    runs = [("f64", torch.float64, "f64", "kbc"), ("f32", torch.float32, "f32", "kbc"),
+            ("bf16_shifted", torch.float32, "bf16_shifted", "kbc"),
+            ("f16_shifted", torch.float32, "f16_shifted", "kbc")]
+    if learned:
+        runs += [("learned_f32", torch.float32, "f32", "learned"),
+                 ("learned_bf16_shifted", torch.float32, "bf16_shifted", "learned")]
+    for name, dtype, storage, coll in runs:
         ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
         flow = lt.DecayingTurbulence(ctx, [res] * 3, re, ma, k0=k0, stencil=lt.D3Q27(),
                                      initialize_pressure=False, initialize_fneq=False,
                                      randseed=seed)
-        sim = lt.Simulation(flow, lt.KBCCollision(), [], lt.StreamingStrategy.PRE_STREAMING)
+        collision = (lt.KBCCollision() if coll == "kbc" else
+                     lt.LearnedCollision(lt.LearnedParameters.seeded(seed, out_scale=0.1)))
+        sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
         t0 = time.perf_counter()
         series[name] = _ke_series(sim, flow, steps, every)
         torch.cuda.synchronize()
@@ -203,7 +212,7 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
         del sim, flow
         torch.cuda.empty_cache()
     ref = series["f64"]
-    for name in ("f32", "bf16_shifted", "f16_shifted"):
+    for name in [r[0] for r in runs if r[0] != "f64"]:
         s = series[name]
         ke_err = np.abs(s[:, 1] / ref[:, 1] - 1)
         diss = -np.diff(s[:, 1])

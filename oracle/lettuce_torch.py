@@ -30,7 +30,9 @@ Pinning (what checks each part; see tests/test_oracle.py):
                       a per cell and step (closed form)
   anti-bounce-back    a uniform equilibrium is a fixed point; the wall
                       velocity extrapolation by hand
-  learned collision   zero MLP output = BGK, rates 1/tau = BGK, forced
+  learned collision   zero MLP output = BGK, rates 1/tau = BGK (D2Q9, D3Q27),
+                      rates beta*gamma = KBC, dual basis = inverse moment
+                      matrix (closed form), forced
                       rates against the inference formula; training step's
                       autograd gradient against central differences
 """
@@ -907,11 +909,39 @@ class LearnedParameters:
         return LearnedParameters(w1, b1, w2, b2, w3 * out_scale, b3 * out_scale)
 
 
-class LearnedCollision:
-    """D2Q9 learned collision.
+def d3q27_raw_moments(dtype=torch.float64):
+    """rows x^a y^b z^c (a, b, c in 0..2, index 9a + 3b + c) of the D3Q27
+    velocities and the order a + b + c of each row"""
+    e = torch.tensor(D3Q27.e, dtype=dtype)
+    rows, orders = [], []
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                rows.append(e[:, 0] ** a * e[:, 1] ** b * e[:, 2] ** c)
+                orders.append(a + b + c)
+    return torch.stack(rows), orders
 
-    With df = f - feq, ds the KBC shear part of df and dh = df - ds (which
-    carries only the moments xxy, xyy, xxyy):
+
+def d3q27_second_moments(df):
+    """(xx, yy, zz, xy, xz, yz) raw moments of df [27, ...]"""
+    e = torch.tensor(D3Q27.e, dtype=df.dtype)
+    x, y, z = e[:, 0], e[:, 1], e[:, 2]
+    return [torch.einsum("q,q...->...", w, df) for w in (x * x, y * y, z * z, x * y, x * z, y * z)]
+
+
+class LearnedCollision:
+    """D2Q9 / D3Q27 learned collision.
+
+    D3Q27: inputs T, Nxz, Nyz, Pxy, Pxz, Pyz of df / rho * in_scale; ds the
+    KBC shear part from those raw moments (kbc_collision.py:41-74 with
+    r = 1); the three rates act on the raw moments x^a y^b z^c of dh of order
+    3, of order 4 and of orders 5-6, reconstructed with the dual basis
+    (inverse of the 27 x 27 raw-moment matrix):
+        f' = f - ds / tau - Phi (rate_of_order(k) * M dh)
+    
+
+    D2Q9: with df = f - feq, ds the KBC shear part of df and dh = df - ds
+    (which carries only the moments xxy, xyy, xxyy):
         x   = (T, N, Pxy, xxy, xyy, xxyy)(df) / rho * in_scale   (bf16)
         o   = W3 silu(W2 silu(W1 x + b1) + b2) + b3   (bf16 operands, FP32 sums)
         r_g = 1/tau + o_g                             raw rates of xxy, xyy, xxyy
@@ -938,9 +968,46 @@ class LearnedCollision:
             [-1] + [1] * (x.dim() - 1))
 
     def inputs(self, flow, df, rho):
-        rows = d2q9_moment_rows(df.dtype)
-        m = torch.einsum("kq,q...->k...", rows, df)
+        if isinstance(flow.stencil, D3Q27):
+            xx, yy, zz, xy, xz, yz = d3q27_second_moments(df)
+            m = torch.stack([xx + yy + zz, xx - zz, yy - zz, xy, xz, yz])
+        else:
+            rows = d2q9_moment_rows(df.dtype)
+            m = torch.einsum("kq,q...->k...", rows, df)
         return (m / rho * self.params.in_scale).to(torch.float32)
+
+    def _call_d3q27(self, flow):
+        tau = flow.units.relaxation_parameter_lu
+        feq = flow.equilibrium(flow)
+        df = flow.f - feq
+        xx, yy, zz, xy, xz, yz = d3q27_second_moments(df)
+        T, Nxz, Nyz = xx + yy + zz, xx - zz, yy - zz
+        ds = torch.zeros_like(df)
+        ds[0] = -T
+        ds[1] = ds[2] = (2 * Nxz - Nyz + T) / 6
+        ds[3] = ds[4] = (2 * Nyz - Nxz + T) / 6
+        ds[5] = ds[6] = (-Nxz - Nyz + T) / 6
+        ds[7] = ds[8] = 0.25 * yz
+        ds[9] = ds[10] = -0.25 * yz
+        ds[11] = ds[12] = 0.25 * xz
+        ds[13] = ds[14] = -0.25 * xz
+        ds[15] = ds[16] = 0.25 * xy
+        ds[17] = ds[18] = -0.25 * xy
+        dh = df - ds
+        M, orders = d3q27_raw_moments(df.dtype)
+        phi = torch.linalg.inv(M)
+        m = torch.einsum("kq,q...->k...", M, dh)
+        omega, fallback = self.rates(flow)
+        group = torch.tensor([-1 if o <= 2 else (0 if o == 3 else (1 if o == 4 else 2))
+                              for o in orders])
+        w = torch.zeros_like(m)
+        for k in range(27):
+            if group[k] >= 0:
+                w[k] = omega[group[k]] * m[k]
+        f_learned = flow.f - ds / tau - torch.einsum("qk,k...->q...", phi, w)
+        if not fallback.any():
+            return f_learned
+        return torch.where(fallback[None], self.kbc(flow), f_learned)
 
     def rates(self, flow):
         """(clamped rates [3, *res], fallback mask [*res])"""
@@ -957,6 +1024,8 @@ class LearnedCollision:
         return raw.clamp(p.rate_lo, p.rate_hi), fallback
 
     def __call__(self, flow):
+        if isinstance(flow.stencil, D3Q27):
+            return self._call_d3q27(flow)
         assert isinstance(flow.stencil, D2Q9)
         tau = flow.units.relaxation_parameter_lu
         feq = flow.equilibrium(flow)

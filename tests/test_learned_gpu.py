@@ -99,3 +99,56 @@ def test_learned_channel_cylinder_vs_oracle(storage):
     lt.Simulation(flow, lt.LearnedCollision(p), [], lt.StreamingStrategy.PRE_STREAMING)(100)
     err = rel_err(flow.f.cpu().numpy(), oflow.f.numpy())
     assert err <= (1e-5 if storage == "f32" else 2e-3)
+
+
+# ------------------------------------------------------------------ D3Q27
+def _f27(res=(6, 8, 64), seed=3, amp=0.05):
+    g = torch.Generator().manual_seed(seed)
+    res = list(res)
+    oflow = _Grid(O.D3Q27(), res, F64)
+    u = 0.05 * (torch.rand([3] + res, generator=g, dtype=F64) - 0.5)
+    rho = 1 + 0.02 * (torch.rand([1] + res, generator=g, dtype=F64) - 0.5)
+    f0 = oflow.equilibrium(oflow, rho=rho, u=u)
+    return (f0 * (1 + amp * (torch.rand(f0.shape, generator=g, dtype=F64) * 2 - 1))).float().numpy()
+
+
+def test_d3q27_zero_mlp_is_bgk():
+    flow = ArrayFlow(ctx(F32), _f27(), lt.D3Q27())
+    tau = flow.units.relaxation_parameter_lu
+    out = lt.LearnedCollision(zero_params())(flow).cpu().numpy()
+    ref = lt.BGKCollision(tau)(flow).cpu().numpy()
+    assert rel_err(out, ref) < 2e-6
+
+
+def test_d3q27_unstable_rates_fall_back_to_kbc():
+    p = zero_params()
+    p.b3 = torch.full((3,), 5.0)
+    flow = ArrayFlow(ctx(F32), _f27(seed=4), lt.D3Q27())
+    out = lt.LearnedCollision(p)(flow).cpu().numpy()
+    ref = lt.KBCCollision()(flow).cpu().numpy()
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_d3q27_learned_collision_vs_oracle(seed):
+    p, op = pair(seed, out_scale=0.3)
+    f0 = _f27(seed=5 + seed)
+    oflow = _Grid(O.D3Q27(), list(f0.shape[1:]), F32, f0)
+    ref = O.LearnedCollision(op)(oflow).numpy()
+    flow = ArrayFlow(ctx(F32), f0, lt.D3Q27())
+    out = lt.LearnedCollision(p)(flow).cpu().numpy()
+    assert rel_err(out, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16_shifted"])
+def test_d3q27_learned_30_steps_vs_oracle(storage):
+    """pull streaming + the learned collision for 30 steps; FP32 storage
+    against the oracle, shifted BF16 storage against the native FP32 run"""
+    p, op = pair(0, out_scale=0.1)
+    f0 = _f27(seed=8, amp=0.01)
+    oflow = _Grid(O.D3Q27(), list(f0.shape[1:]), F32, f0)
+    O.Simulation(oflow, O.LearnedCollision(op), [], O.StreamingStrategy.PRE_STREAMING)(30)
+    flow = ArrayFlow(ctx(F32, storage), f0, lt.D3Q27())
+    lt.Simulation(flow, lt.LearnedCollision(p), [], lt.StreamingStrategy.PRE_STREAMING)(30)
+    err = rel_err(flow.f.cpu().numpy(), oflow.f.numpy())
+    assert err <= (1e-5 if storage == "f32" else 2e-2), err

@@ -134,3 +134,85 @@ def test_train_step_autograd_matches_central_differences(clamped):
     if clamped:
         assert float(th.grad[-3]) == 0.0
         assert torch.all(th.grad[-3 - 96:-3 - 64] == 0)
+
+
+# ------------------------------------------------------------------ D3Q27
+def _grid27(n=4, seed=3, amp=0.05):
+    g = torch.Generator().manual_seed(seed)
+    shape = [n, n, n]
+    u = 0.05 * (torch.rand([3] + shape, generator=g, dtype=F64) - 0.5)
+    rho = 1 + 0.02 * (torch.rand(shape, generator=g, dtype=F64) - 0.5)
+    e = torch.tensor(O.D3Q27.e, dtype=F64)
+    w = torch.tensor(O.D3Q27.w, dtype=F64).reshape([27, 1, 1, 1])
+    eu = torch.einsum("qd,d...->q...", e, u)
+    feq = w * rho * (1 + 3 * eu - 1.5 * (u * u).sum(0) + 4.5 * eu * eu)
+    r = torch.rand(feq.shape, generator=g, dtype=F64) * 2 - 1
+    return feq * (1 + amp * r)
+
+
+class _Flow27(O.Flow):
+    def __init__(self, f, tau):
+        super().__init__(O.Context("cpu", F64), list(f.shape[1:]), _Units(tau), O.D3Q27())
+        self.f = f.clone()
+
+
+def test_d3q27_dual_basis_is_the_tensor_product_of_1d_duals():
+    """inverse of the raw-moment matrix = products of the 1D duals of
+    (1, x, x^2) at x = -1, 0, 1: (x^2 - x)/2, 1 - x^2, (x^2 + x)/2"""
+    M, orders = O.d3q27_raw_moments()
+    phi = torch.linalg.inv(M)
+    W = {-1: [0.0, -0.5, 0.5], 0: [1.0, 0.0, -1.0], 1: [0.0, 0.5, 0.5]}
+    e = O.D3Q27.e
+    closed = torch.tensor([[W[e[q][0]][k // 9] * W[e[q][1]][(k // 3) % 3] * W[e[q][2]][k % 3]
+                            for k in range(27)] for q in range(27)], dtype=F64)
+    torch.testing.assert_close(phi, closed, rtol=0, atol=1e-14)
+    assert orders.count(3) == 7 and orders.count(4) == 6 and orders.count(5) == 3
+
+
+def test_d3q27_shear_part_carries_exactly_the_second_order_moments():
+    """ds (the KBC shear part from raw moments) has the second-order moments
+    of df and no moment of order 0, 1 or >= 3, so dh = df - ds lives in the
+    orders 3..6 that the three rates act on"""
+    f = _grid27()
+    flow = _Flow27(f, 0.6)
+    params = O.LearnedParameters.seeded(0)
+    params.rate_lo = 0.0
+    lc = O.LearnedCollision(params)
+    lc.force_rates = [0.0, 0.0, 0.0]
+    tau = 0.6
+    # with all rates 0 the update is f - ds / tau: recover ds
+    ds = (f - lc(flow)) * tau
+    feq = flow.equilibrium(flow)
+    M, orders = O.d3q27_raw_moments()
+    m_ds = torch.einsum("kq,q...->k...", M, ds)
+    m_df = torch.einsum("kq,q...->k...", M, f - feq)
+    for k, o in enumerate(orders):
+        if o == 2:
+            torch.testing.assert_close(m_ds[k], m_df[k], rtol=0, atol=1e-14)
+        else:
+            assert float(m_ds[k].abs().max()) <= 1e-14, (k, o)
+
+
+def test_d3q27_rates_one_over_tau_is_bgk_and_beta_gamma_is_kbc():
+    tau = 0.57
+    f = _grid27(seed=5)
+    flow = _Flow27(f, tau)
+    lc = O.LearnedCollision(O.LearnedParameters.seeded(0))
+    lc.force_rates = [1 / tau] * 3
+    torch.testing.assert_close(lc(flow), O.BGKCollision(tau)(flow), rtol=0, atol=1e-14)
+    kbc = O.KBCCollision()
+    kbc.force_gamma = 1.7
+    beta = 1 / (2 * tau)
+    lc.force_rates = [beta * 1.7] * 3
+    torch.testing.assert_close(lc(flow), kbc(flow), rtol=0, atol=1e-13)
+
+
+def test_d3q27_learned_conserves_mass_and_momentum():
+    f = _grid27(seed=7)
+    flow = _Flow27(f, 0.6)
+    lc = O.LearnedCollision(O.LearnedParameters.seeded(2, out_scale=0.5))
+    post = lc(flow)
+    e = torch.tensor(O.D3Q27.e, dtype=F64)
+    torch.testing.assert_close(post.sum(0), f.sum(0), rtol=0, atol=1e-14)
+    torch.testing.assert_close(torch.einsum("qd,q...->d...", e, post),
+                               torch.einsum("qd,q...->d...", e, f), rtol=0, atol=1e-14)
