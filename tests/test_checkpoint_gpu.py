@@ -69,3 +69,21 @@ def test_checkpoint_mismatches_are_rejected(tmp_path):
         other.load_checkpoint(path)
     other.load_checkpoint(path, check_units=False)
     assert other.i == 0 and torch.equal(other.f, a.f)
+
+
+def test_snapshot_reporter_feeds_the_dataset(tmp_path):
+    """SnapshotReporter every 5 steps -> LettuceDataset pairs (lettuce's
+    HDF5Reporter / LettuceDataset, util/datautils.py:17-156)"""
+    flow = _flow("f16_shifted")
+    base = str(tmp_path / "snaps")
+    rep = lt.SnapshotReporter(flow, lt.BGKCollision(0.6), interval=5, filebase=base,
+                              metadata={"case": "tgv"})
+    sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [rep],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    sim(20)
+    ds = lt.LettuceDataset(base, target=True, skip_idx_to_target=1)
+    assert ds.get_attr("steps") == [0, 5, 10, 15, 20] and len(ds) == 4
+    assert ds.get_attr("metadata") == {"case": "tgv"} and ds.get_attr("storage") == "f16_shifted"
+    f, target, idx = ds[3]
+    assert idx == 3 and torch.equal(target, flow.f.cpu())
+    assert f.shape == target.shape == (9, 32, 32)
