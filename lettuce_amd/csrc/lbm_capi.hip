@@ -169,7 +169,8 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
-  return pick_tb2_kernel(a->stencil, a->storage, a->collision);
+  return pick_tb2_kernel(a->stencil, a->storage, a->collision,
+                         env_int("LETTUCE_AMD_TB2_PREFETCH", 0) != 0);
 }
 
 // pairs of steps fused into one pass by the two-step kernel.  After nsteps

@@ -48,7 +48,7 @@ __host__ __device__ constexpr int tb_base(int q, int which) {
   return b;
 }
 
-template <class S, class ST, int COLL>
+template <class S, class ST, int COLL, bool PREFETCH>
 __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   using C = typename ST::C;
   using T = typename ST::T;
@@ -95,17 +95,37 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) h1[q] = h2[q] = C(0);
 
-  for (int it = 0; it < cx + 2; ++it) {
-    // ---- A: S1 of plane xa = x0 - 1 + it (ring counter it) ----
+  // with PREFETCH, the populations S1 of the next plane pulls are loaded
+  // into `raw` while this plane's collisions run
+  T raw[S::Q];
+  auto plane_offsets = [&](int it, int (&ax)[3]) {
     int xa = x0 - 1 + it;
     xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
-    int ax[3];
     axis_offsets(xa, p.n0, p.s0, ax);
-    if (active) {
+  };
+  auto load_raw = [&](int it) {
+    int ax[3];
+    plane_offsets(it, ax);
 #pragma unroll
-      for (int q = 0; q < S::Q; ++q) {
-        const C sh = shift_of<ST>(S::w[q], p);
-        cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)], sh);
+    for (int q = 0; q < S::Q; ++q)
+      raw[q] = fin[q * p.sq + ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)]];
+  };
+  if (PREFETCH && active) load_raw(0);
+  for (int it = 0; it < cx + 2; ++it) {
+    // ---- A: S1 of plane xa = x0 - 1 + it (ring counter it) ----
+    if (active) {
+      if (PREFETCH) {
+#pragma unroll
+        for (int q = 0; q < S::Q; ++q) cur[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+        if (it + 1 < cx + 2) load_raw(it + 1);
+      } else {
+        int ax[3];
+        plane_offsets(it, ax);
+#pragma unroll
+        for (int q = 0; q < S::Q; ++q) {
+          const C sh = shift_of<ST>(S::w[q], p);
+          cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)], sh);
+        }
       }
       collide<S, C, COLL>(cur, cp);
 #pragma unroll
@@ -168,32 +188,37 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   }
 }
 
+template <class S, class ST, int COLL>
+TbFn pick_tb2_pf(bool prefetch) {
+  return prefetch ? tb2_kernel<S, ST, COLL, true> : tb2_kernel<S, ST, COLL, false>;
+}
+
 template <class S, class ST>
-TbFn pick_tb2_coll(int coll) {
+TbFn pick_tb2_coll(int coll, bool prefetch) {
   switch (coll) {
-    case LBM_COLL_NONE: return tb2_kernel<S, ST, LBM_COLL_NONE>;
-    case LBM_COLL_BGK: return tb2_kernel<S, ST, LBM_COLL_BGK>;
-    case LBM_COLL_TRT: return tb2_kernel<S, ST, LBM_COLL_TRT>;
+    case LBM_COLL_NONE: return pick_tb2_pf<S, ST, LBM_COLL_NONE>(prefetch);
+    case LBM_COLL_BGK: return pick_tb2_pf<S, ST, LBM_COLL_BGK>(prefetch);
+    case LBM_COLL_TRT: return pick_tb2_pf<S, ST, LBM_COLL_TRT>(prefetch);
     case LBM_COLL_KBC:
-      if constexpr (S::ID == LBM_D3Q27) return tb2_kernel<S, ST, LBM_COLL_KBC>;
+      if constexpr (S::ID == LBM_D3Q27) return pick_tb2_pf<S, ST, LBM_COLL_KBC>(prefetch);
       else return nullptr;
   }
   return nullptr;
 }
 
 template <class S>
-TbFn pick_tb2(int storage, int coll) {
+TbFn pick_tb2(int storage, int coll, bool prefetch) {
   switch (storage) {
-    case LBM_F32: return pick_tb2_coll<S, StF32>(coll);
-    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll);
-    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll);
+    case LBM_F32: return pick_tb2_coll<S, StF32>(coll, prefetch);
+    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, prefetch);
+    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, prefetch);
   }
   return nullptr;
 }
 
-TbFn pick_tb2_kernel(int stencil, int storage, int coll) {
-  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll);
-  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll);
+TbFn pick_tb2_kernel(int stencil, int storage, int coll, bool prefetch) {
+  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll, prefetch);
+  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll, prefetch);
   return nullptr;
 }
 
