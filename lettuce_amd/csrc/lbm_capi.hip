@@ -161,7 +161,9 @@ static int block_order(const KParams& p, int64_t nb) {
 // 79.6k); D3Q27 KBC loses (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave
 // one block per CU).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it
 // applies; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64) and
-// LETTUCE_AMD_TB2_XCD=0 the plain tile order.
+// LETTUCE_AMD_TB2_XCD=0 the plain tile order.  LETTUCE_AMD_TB2_PREFETCH=1
+// selects a variant that loads the next plane during phase B; its 116 VGPRs
+// leave one block per CU and it measured slower (53.0k vs 57.8k MLUPS).
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
