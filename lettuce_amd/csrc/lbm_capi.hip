@@ -161,9 +161,10 @@ static int block_order(const KParams& p, int64_t nb) {
 // 79.6k); D3Q27 KBC loses (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave
 // one block per CU).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it
 // applies; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64) and
-// LETTUCE_AMD_TB2_XCD=0 the plain tile order.  LETTUCE_AMD_TB2_PREFETCH=1
-// selects a variant that loads the next plane during phase B; its 116 VGPRs
-// leave one block per CU and it measured slower (53.0k vs 57.8k MLUPS).
+// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=1
+// the variant with one barrier per plane (deeper LDS rings).  A variant that
+// loaded the next plane during phase B measured slower (53.0k vs 57.8k
+// MLUPS: its 116 VGPRs left one block per CU) and was removed.
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
@@ -172,7 +173,7 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
   return pick_tb2_kernel(a->stencil, a->storage, a->collision,
-                         env_int("LETTUCE_AMD_TB2_PREFETCH", 0) != 0);
+                         env_int("LETTUCE_AMD_TB2_BARRIERS", 2) == 1 ? 1 : 2);
 }
 
 // pairs of steps fused into one pass by the two-step kernel.  After nsteps

@@ -19,8 +19,8 @@ struct TbParams {
 };
 using TbFn = void (*)(const TbParams);
 
-// two-step kernel for (stencil, storage, collision), or nullptr; prefetch
-// selects the variant that loads the next plane during phase B
-TbFn pick_tb2_kernel(int stencil, int storage, int coll, bool prefetch);
+// two-step kernel for (stencil, storage, collision), or nullptr, with one
+// or two block-wide barriers per plane
+TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers);
 
 }  // namespace lbm

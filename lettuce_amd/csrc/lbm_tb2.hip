@@ -17,7 +17,8 @@
 //                  lane shift (DPP), the row ends from the z-edge cells;
 //  e_y != 0        LDS, where each such population of S1 lives in a ring of
 //                  2 + e_x planes (it is read by the second step 1 + e_x
-//                  planes after it was computed).
+//                  planes after it was computed), 3 + e_x with one barrier
+//                  per plane.
 // S1 values are rounded to the storage format as a stored step rounds them,
 // so the result equals two step_kernel launches up to the rounding of the
 // collision arithmetic (bit for bit without collision).  A variant that
@@ -32,29 +33,34 @@ template <class S>
 __host__ __device__ constexpr int tb_kind(int q) {
   return go<S>(q, 1) != 0 ? 2 : (go<S>(q, 2) != 0 ? 1 : 0);
 }
-template <class S>
+// ring depth of population q: S1 of counter c is read by phase B of
+// iterations up to c + 1 + e_x; with a second barrier after phase B the
+// slot can be rewritten at iteration c + 2 + e_x (depth 2 + e_x), with only
+// the barrier after phase A at c + 3 + e_x (depth 3 + e_x: a wavefront that
+// runs ahead writes phase A of the next plane while others still read)
+template <class S, int NBAR>
 __host__ __device__ constexpr int tb_ring(int q) {
-  return 2 + go<S>(q, 0);
+  return (NBAR == 2 ? 2 : 3) + go<S>(q, 0);
 }
 // first ring slot of population q among the populations selected by `which`
 // (0: e_y != 0; 1: e_z > 0; 2: e_z < 0) and the total slot count (q == Q)
-template <class S>
+template <class S, int NBAR>
 __host__ __device__ constexpr int tb_base(int q, int which) {
   int b = 0;
   for (int p = 0; p < q && p < S::Q; ++p) {
     const bool sel = which == 0 ? go<S>(p, 1) != 0 : (which == 1 ? go<S>(p, 2) > 0 : go<S>(p, 2) < 0);
-    if (sel) b += tb_ring<S>(p);
+    if (sel) b += tb_ring<S, NBAR>(p);
   }
   return b;
 }
 
-template <class S, class ST, int COLL, bool PREFETCH>
+template <class S, class ST, int COLL, int NBAR>
 __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   using C = typename ST::C;
   using T = typename ST::T;
-  constexpr int NS2 = tb_base<S>(S::Q, 0);
-  constexpr int NSL = tb_base<S>(S::Q, 1);
-  constexpr int NSR = tb_base<S>(S::Q, 2);
+  constexpr int NS2 = tb_base<S, NBAR>(S::Q, 0);
+  constexpr int NSL = tb_base<S, NBAR>(S::Q, 1);
+  constexpr int NSR = tb_base<S, NBAR>(S::Q, 2);
   __shared__ T lds2[NS2][TB_ROWS][64];
   __shared__ T ldsL[NSL][TB_ROWS];
   __shared__ T ldsR[NSR][TB_ROWS];
@@ -95,37 +101,17 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) h1[q] = h2[q] = C(0);
 
-  // with PREFETCH, the populations S1 of the next plane pulls are loaded
-  // into `raw` while this plane's collisions run
-  T raw[S::Q];
-  auto plane_offsets = [&](int it, int (&ax)[3]) {
-    int xa = x0 - 1 + it;
-    xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
-    axis_offsets(xa, p.n0, p.s0, ax);
-  };
-  auto load_raw = [&](int it) {
-    int ax[3];
-    plane_offsets(it, ax);
-#pragma unroll
-    for (int q = 0; q < S::Q; ++q)
-      raw[q] = fin[q * p.sq + ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)]];
-  };
-  if (PREFETCH && active) load_raw(0);
   for (int it = 0; it < cx + 2; ++it) {
     // ---- A: S1 of plane xa = x0 - 1 + it (ring counter it) ----
     if (active) {
-      if (PREFETCH) {
+      int xa = x0 - 1 + it;
+      xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
+      int ax[3];
+      axis_offsets(xa, p.n0, p.s0, ax);
 #pragma unroll
-        for (int q = 0; q < S::Q; ++q) cur[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
-        if (it + 1 < cx + 2) load_raw(it + 1);
-      } else {
-        int ax[3];
-        plane_offsets(it, ax);
-#pragma unroll
-        for (int q = 0; q < S::Q; ++q) {
-          const C sh = shift_of<ST>(S::w[q], p);
-          cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)], sh);
-        }
+      for (int q = 0; q < S::Q; ++q) {
+        const C sh = shift_of<ST>(S::w[q], p);
+        cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)], sh);
       }
       collide<S, C, COLL>(cur, cp);
 #pragma unroll
@@ -133,13 +119,13 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         const C sh = shift_of<ST>(S::w[q], p);
         const T t = ST::enc(cur[q], sh);
         cur[q] = ST::dec(t, sh);
-        const int slot_of = it % tb_ring<S>(q);
+        const int slot_of = it % tb_ring<S, NBAR>(q);
         if (!edge) {
-          if (tb_kind<S>(q) == 2) lds2[tb_base<S>(q, 0) + slot_of][row][lane] = t;
+          if (tb_kind<S>(q) == 2) lds2[tb_base<S, NBAR>(q, 0) + slot_of][row][lane] = t;
         } else if (side == 0) {
-          if (go<S>(q, 2) > 0) ldsL[tb_base<S>(q, 1) + slot_of][row] = t;
+          if (go<S>(q, 2) > 0) ldsL[tb_base<S, NBAR>(q, 1) + slot_of][row] = t;
         } else {
-          if (go<S>(q, 2) < 0) ldsR[tb_base<S>(q, 2) + slot_of][row] = t;
+          if (go<S>(q, 2) < 0) ldsR[tb_base<S, NBAR>(q, 2) + slot_of][row] = t;
         }
       }
     }
@@ -152,7 +138,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) {
         const int ex = go<S>(q, 0), ey = go<S>(q, 1), ez = go<S>(q, 2);
-        const int slot = (it - 1 - ex) % tb_ring<S>(q);
+        const int slot = (it - 1 - ex) % tb_ring<S, NBAR>(q);
         const C sh = shift_of<ST>(S::w[q], p);
         const C reg = ex < 0 ? cur[q] : (ex == 0 ? h1[q] : h2[q]);
         if (tb_kind<S>(q) == 0) {
@@ -160,17 +146,17 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         } else if (tb_kind<S>(q) == 1) {
           if (ez > 0) {
             g[q] = wave_shift<+1>(reg);
-            if (lane == 0) g[q] = ST::dec(ldsL[tb_base<S>(q, 1) + slot][row], sh);
+            if (lane == 0) g[q] = ST::dec(ldsL[tb_base<S, NBAR>(q, 1) + slot][row], sh);
           } else {
             g[q] = wave_shift<-1>(reg);
-            if (lane == 63) g[q] = ST::dec(ldsR[tb_base<S>(q, 2) + slot][row], sh);
+            if (lane == 63) g[q] = ST::dec(ldsR[tb_base<S, NBAR>(q, 2) + slot][row], sh);
           }
         } else {
           const int rr = row - ey, ll = lane - ez;
           T t;
-          if (ll < 0) t = ldsL[tb_base<S>(q, 1) + slot][rr];
-          else if (ll > 63) t = ldsR[tb_base<S>(q, 2) + slot][rr];
-          else t = lds2[tb_base<S>(q, 0) + slot][rr][ll];
+          if (ll < 0) t = ldsL[tb_base<S, NBAR>(q, 1) + slot][rr];
+          else if (ll > 63) t = ldsR[tb_base<S, NBAR>(q, 2) + slot][rr];
+          else t = lds2[tb_base<S, NBAR>(q, 0) + slot][rr][ll];
           g[q] = ST::dec(t, sh);
         }
       }
@@ -184,41 +170,41 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
       h2[q] = h1[q];
       h1[q] = cur[q];
     }
-    __syncthreads();
+    if (NBAR == 2) __syncthreads();
   }
 }
 
 template <class S, class ST, int COLL>
-TbFn pick_tb2_pf(bool prefetch) {
-  return prefetch ? tb2_kernel<S, ST, COLL, true> : tb2_kernel<S, ST, COLL, false>;
+TbFn pick_tb2_bar(int barriers) {
+  return barriers == 1 ? tb2_kernel<S, ST, COLL, 1> : tb2_kernel<S, ST, COLL, 2>;
 }
 
 template <class S, class ST>
-TbFn pick_tb2_coll(int coll, bool prefetch) {
+TbFn pick_tb2_coll(int coll, int barriers) {
   switch (coll) {
-    case LBM_COLL_NONE: return pick_tb2_pf<S, ST, LBM_COLL_NONE>(prefetch);
-    case LBM_COLL_BGK: return pick_tb2_pf<S, ST, LBM_COLL_BGK>(prefetch);
-    case LBM_COLL_TRT: return pick_tb2_pf<S, ST, LBM_COLL_TRT>(prefetch);
+    case LBM_COLL_NONE: return pick_tb2_bar<S, ST, LBM_COLL_NONE>(barriers);
+    case LBM_COLL_BGK: return pick_tb2_bar<S, ST, LBM_COLL_BGK>(barriers);
+    case LBM_COLL_TRT: return pick_tb2_bar<S, ST, LBM_COLL_TRT>(barriers);
     case LBM_COLL_KBC:
-      if constexpr (S::ID == LBM_D3Q27) return pick_tb2_pf<S, ST, LBM_COLL_KBC>(prefetch);
+      if constexpr (S::ID == LBM_D3Q27) return pick_tb2_bar<S, ST, LBM_COLL_KBC>(barriers);
       else return nullptr;
   }
   return nullptr;
 }
 
 template <class S>
-TbFn pick_tb2(int storage, int coll, bool prefetch) {
+TbFn pick_tb2(int storage, int coll, int barriers) {
   switch (storage) {
-    case LBM_F32: return pick_tb2_coll<S, StF32>(coll, prefetch);
-    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, prefetch);
-    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, prefetch);
+    case LBM_F32: return pick_tb2_coll<S, StF32>(coll, barriers);
+    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, barriers);
+    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, barriers);
   }
   return nullptr;
 }
 
-TbFn pick_tb2_kernel(int stencil, int storage, int coll, bool prefetch) {
-  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll, prefetch);
-  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll, prefetch);
+TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers) {
+  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll, barriers);
+  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll, barriers);
   return nullptr;
 }
 

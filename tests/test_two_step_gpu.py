@@ -70,8 +70,10 @@ ULPS = {"f32": 4 * 0.4 * 2.0 ** -23, "f16_shifted": 4 * 0.05 * 2.0 ** -10,
 @pytest.mark.parametrize("storage,layout", [("f32", "soa"), ("f32", "rows"),
                                             ("f16_shifted", "soa"), ("bf16_shifted", "rows")])
 @pytest.mark.parametrize("coll", ["bgk", "trt", "none"])
-def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, coll,
+@pytest.mark.parametrize("barriers", ["2", "1"])
+def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, coll, barriers,
                                                      monkeypatch):
+    monkeypatch.setenv("LETTUCE_AMD_TB2_BARRIERS", barriers)
     f0 = _f0(res)
     make = {"bgk": lambda: lt.BGKCollision(0.62), "trt": lambda: lt.TRTCollision(0.7, 0.9),
             "none": lambda: lt.NoCollision()}[coll]
