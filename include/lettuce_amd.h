@@ -182,8 +182,7 @@ int lbm_run(const lbm_step_args* args, int64_t nsteps, void* stream);
 /* How lbm_run would execute `nsteps` steps, without launching anything:
  * plan[0] = launches of the two-step kernel (two fused steps each: periodic
  * D3Q19 / D3Q27 without boundaries, pull streaming, contiguous extent a
- * multiple of 64, i1 extent a multiple of 8; by default for D3Q19 with FP32
- * storage only),
+ * multiple of 64, i1 extent a multiple of 8; by default for D3Q19),
  * plan[1] = single-step launches.  No lettuce counterpart; for reporting. */
 int lbm_run_plan(const lbm_step_args* args, int64_t nsteps, int64_t* plan);
 

@@ -155,25 +155,26 @@ static int block_order(const KParams& p, int64_t nb) {
 
 // the two-step kernel for this step, if one applies: periodic D3Q19 or
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
-// TB_Y.  Used by default for D3Q19 with FP32 storage, where the single step
-// is bound by HBM (measured on MI355X, BGK 256^3: 38.8k -> 58.0k MLUPS).
-// The 16-bit formats stream half the bytes and gain nothing (FP16 79.0k ->
-// 79.6k); D3Q27 KBC loses (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave
-// one block per CU).  LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it
-// applies; LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64) and
-// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=1
-// the variant with one barrier per plane (deeper LDS rings).  A variant that
-// loaded the next plane during phase B measured slower (53.0k vs 57.8k
-// MLUPS: its 116 VGPRs left one block per CU) and was removed.
+// TB_Y.  Used by default for D3Q19, with one barrier per plane (measured on
+// MI355X, BGK TGV 256^3: FP32 38.8k -> 58.0k MLUPS, FP16-shifted 79.0k ->
+// 83.7k; with two barriers per plane FP16 fell to 78.4k).  D3Q27 KBC loses
+// (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave one block per CU).
+// LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it applies;
+// LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64),
+// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=2
+// the variant with two barriers per plane and shallower LDS rings.  A
+// variant that loaded the next plane during phase B measured slower
+// (53.0k vs 57.8k MLUPS: its 116 VGPRs left one block per CU) and was
+// removed.
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
-  if (mode == 1 && (a->storage != LBM_F32 || a->stencil != LBM_D3Q19)) return nullptr;
+  if (mode == 1 && a->stencil != LBM_D3Q19) return nullptr;
   if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
   return pick_tb2_kernel(a->stencil, a->storage, a->collision,
-                         env_int("LETTUCE_AMD_TB2_BARRIERS", 2) == 1 ? 1 : 2);
+                         env_int("LETTUCE_AMD_TB2_BARRIERS", 1) == 2 ? 2 : 1);
 }
 
 // pairs of steps fused into one pass by the two-step kernel.  After nsteps

@@ -70,7 +70,7 @@ ULPS = {"f32": 4 * 0.4 * 2.0 ** -23, "f16_shifted": 4 * 0.05 * 2.0 ** -10,
 @pytest.mark.parametrize("storage,layout", [("f32", "soa"), ("f32", "rows"),
                                             ("f16_shifted", "soa"), ("bf16_shifted", "rows")])
 @pytest.mark.parametrize("coll", ["bgk", "trt", "none"])
-@pytest.mark.parametrize("barriers", ["2", "1"])
+@pytest.mark.parametrize("barriers", ["1", "2"])
 def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, coll, barriers,
                                                      monkeypatch):
     monkeypatch.setenv("LETTUCE_AMD_TB2_BARRIERS", barriers)
@@ -107,11 +107,11 @@ def test_two_step_kernel_matches_the_oracle(layout, monkeypatch):
 
 
 def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
-    """FP32 only by default; no two-step launch with boundaries, push
-    streaming, or extents that do not tile"""
+    """D3Q19 in every storage format by default; not D3Q27 by default; no
+    two-step launch with push streaming or extents that do not tile"""
     monkeypatch.delenv("LETTUCE_AMD_TB2", raising=False)
     f0 = _f0([4, 16, 64])
-    for storage, pairs in (("f32", 4), ("f16_shifted", 0)):
+    for storage, pairs in (("f32", 4), ("f16_shifted", 4), ("bf16_shifted", 4)):
         c = lt.Context("cuda:0", dtype=F32, storage=storage)
         sim = lt.Simulation(_Flow(c, f0.astype(np.float32)), lt.BGKCollision(0.6), [],
                             lt.StreamingStrategy.PRE_STREAMING)
@@ -122,6 +122,10 @@ def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
     assert sim._collide_and_stream.plan(8) == (0, 8)
     f1 = _f0([4, 12, 64])
     sim = lt.Simulation(_Flow(c, f1.astype(np.float32)), lt.BGKCollision(0.6), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (0, 8)
+    f2 = _f0([4, 16, 64], stencil=O.D3Q27)
+    sim = lt.Simulation(_Flow(c, f2.astype(np.float32)), lt.KBCCollision(), [],
                         lt.StreamingStrategy.PRE_STREAMING)
     assert sim._collide_and_stream.plan(8) == (0, 8)
 
