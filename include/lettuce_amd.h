@@ -316,6 +316,51 @@ int lbm_learned_train_backward(int32_t storage, const int64_t n[3], double tau,
 int lbm_unpull(int32_t storage, const int64_t n[3], const void* mu, void* lam,
                void* stream);
 
+/* ---------------------------------------------------------------------------
+ * x-slabs: one device's share of a lattice split along i0 (no lettuce
+ * counterpart: lettuce steps the whole lattice on one device,
+ * lettuce/_simulation.py:241-256 rolls every axis periodically).
+ *
+ * A slab holds planes 0 .. n[0]-1 of a 3D lattice (D3Q19 / D3Q27) that is
+ * periodic along i1 and i2; the planes i0 = -1 and i0 = n[0] live on the
+ * neighbouring slabs.  Per pull step the populations that cross an i0 face
+ * travel as packed halo buffers, each [k][n[1]][n[2]] contiguous elements of
+ * the storage type, k = lbm_halo_planes(stencil) populations (those with
+ * e_x = +1, resp. -1, in stencil order):
+ *   send_hi  the slab's plane n[0]-1, e_x = +1  -> the upper neighbour's recv_lo
+ *   send_lo  the slab's plane 0,      e_x = -1  -> the lower neighbour's recv_hi
+ * The transfer between devices is the caller's (RCCL send/recv, a peer copy).
+ * One slab whose recv_lo / recv_hi are copies of its own send_hi / send_lo
+ * steps exactly like the periodic lattice.
+ * ------------------------------------------------------------------------- */
+enum lbm_slab_part {
+  LBM_SLAB_ALL = 0,      /* interior planes, then the two face planes       */
+  LBM_SLAB_INTERIOR = 1, /* planes 1 .. n[0]-2 (needs no halo: overlaps the transfer) */
+  LBM_SLAB_FACES = 2     /* planes 0 and n[0]-1 (reads recv_*, writes send_*) */
+};
+
+/* Populations that cross one i0 face: 5 (D3Q19), 9 (D3Q27); -1 otherwise. */
+int64_t lbm_halo_planes(int32_t stencil);
+
+/* send_lo / send_hi <- the leaving populations of planes 0 / n[0]-1 of f (raw
+ * storage bits), for a state that no lbm_slab_step produced (the initial
+ * condition, a restart).  f uses `stride` as in lbm_step_args. */
+int lbm_halo_pack(int32_t stencil, int32_t storage, const int64_t n[3],
+                  const int64_t stride[4], const void* f, void* send_lo,
+                  void* send_hi, void* stream);
+
+/* One pull step (args->streaming == LBM_PRE_STREAMING, no cell_class, any
+ * collision but LEARNED) of the slab f_in -> f_out, or the part of it that
+ * `part` selects.  The face planes pull the entering populations from
+ * recv_lo / recv_hi and write the leaving ones of f_out to send_lo / send_hi
+ * (the next step's halos), so no pack pass runs between steps.  The four
+ * halo buffers must be distinct; they are ignored for LBM_SLAB_INTERIOR.  The
+ * arithmetic is that of lbm_step; the caller swaps f_in / f_out after the
+ * face part, and must not overwrite recv_* before the face part has run. */
+int lbm_slab_step(const lbm_step_args* args, int32_t part,
+                  const void* recv_lo, const void* recv_hi,
+                  void* send_lo, void* send_hi, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

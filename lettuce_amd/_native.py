@@ -30,6 +30,7 @@ COLL_NONE, COLL_BGK, COLL_TRT, COLL_KBC, COLL_LEARNED, COLL_BGK_GUO = 0, 1, 2, 3
 OP_COLLIDE, OP_BOUNCE_BACK, OP_EQUILIBRIUM, OP_OUTLET, OP_KEEP = 0, 1, 2, 3, 4
 OP_ANTI_BOUNCE_BACK = 5
 CLASS_SOLID, CLASS_NEAR_SOLID, CLASS_BC_FIELD = 0x01, 0x02, 0x04
+SLAB_ALL, SLAB_INTERIOR, SLAB_FACES = 0, 1, 2
 
 STENCIL_IDS = {"D2Q9": D2Q9, "D3Q19": D3Q19, "D3Q27": D3Q27}
 STORAGE_IDS = {"f32": F32, "f64": F64, "f16_shifted": F16_SHIFTED,
@@ -103,6 +104,10 @@ _SIGNATURES = {
     "lbm_learned_train_backward": ([_i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp, _i32,
                                     _vp, _vp, _vp], ctypes.c_int),
     "lbm_unpull": ([_i32, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_halo_planes": ([_i32], _i64),
+    "lbm_halo_pack": ([_i32, _i32, _I64_3, _I64_4, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_slab_step": ([ctypes.POINTER(LbmStepArgs), _i32, _vp, _vp, _vp, _vp, _vp],
+                      ctypes.c_int),
 }
 
 _lock = threading.Lock()

@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "lbm_step_impl.h"
+#include "lbm_slab.h"
 #include "lbm_tb2.h"
 
 namespace lbm {
@@ -186,7 +187,10 @@ static int64_t two_step_pairs(TbFn tb, int64_t nsteps) {
   return pairs;
 }
 
-static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
+// generic: use step_kernel (one cell per thread, block_tile indexing) even
+// where the pair or row kernels are requested
+static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block,
+                        bool generic = false) {
   if (!a) return fail(LBM_E_ARG, "NULL args");
   if (a->abi_version != LBM_ABI_VERSION)
     return fail(LBM_E_VERSION, "abi_version %d != library %d", a->abi_version, LBM_ABI_VERSION);
@@ -255,7 +259,7 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   // two cells per thread where requested, a pair kernel exists and rows
   // have even length
   int64_t row_threads = a->n[2];
-  if (fn && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 1) == 2) {
+  if (fn && !generic && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 1) == 2) {
     const bool nt = env_int("LETTUCE_AMD_NT_STORE", 1) != 0;
     KernelFn pf = nullptr;
     switch (a->stencil) {
@@ -270,7 +274,7 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   }
   // one block per row with aligned loads and lane shifts along i2
   bool row_kernel = false;
-  if (fn && !mask && a->streaming == LBM_PRE_STREAMING && a->n[2] % 64 == 0 && a->n[2] <= 1024 &&
+  if (fn && !generic && !mask && a->streaming == LBM_PRE_STREAMING && a->n[2] % 64 == 0 && a->n[2] <= 1024 &&
       env_int("LETTUCE_AMD_ROW_KERNEL", 0) != 0 && row_threads == a->n[2]) {
     KernelFn rf = nullptr;
     switch (a->stencil) {
@@ -380,6 +384,100 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
     hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
   }
   return check_launch("lbm_run");
+}
+
+int64_t lbm_halo_planes(int32_t stencil) {
+  const int k = halo_planes(stencil);
+  if (k < 0) fail(LBM_E_ARG, "x-slabs are defined for D3Q19 and D3Q27, not stencil %d", stencil);
+  return k;
+}
+
+}  // extern "C"
+
+// the halo planes of an x-slab must be addressable with 32-bit offsets
+static int check_slab_extent(int stencil, const int64_t n[3]) {
+  const int k = halo_planes(stencil);
+  if (k < 0) return fail(LBM_E_UNSUPPORTED, "x-slabs are defined for D3Q19 and D3Q27");
+  if ((int64_t)k * n[1] * n[2] > (int64_t)0x7fffffff)
+    return fail(LBM_E_ARG, "halo planes too large for 32-bit offsets");
+  return LBM_OK;
+}
+
+static bool distinct4(const void* a, const void* b, const void* c, const void* d) {
+  return a != b && a != c && a != d && b != c && b != d && c != d;
+}
+
+extern "C" {
+
+int lbm_halo_pack(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
+                  const void* f, void* send_lo, void* send_hi, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !send_lo || !send_hi) return fail(LBM_E_ARG, "NULL population or halo pointer");
+  if (send_lo == send_hi) return fail(LBM_E_ARG, "send_lo and send_hi must be distinct buffers");
+  rc = check_slab_extent(stencil, n);
+  if (rc) return rc;
+  FaceFn fn = pick_pack_kernel(stencil, storage);
+  if (!fn) return fail(LBM_E_UNSUPPORTED, "no pack kernel for storage %d", storage);
+  KParams p;
+  p.fin = f;
+  p.n0 = (int)n[0];
+  p.n1 = (int)n[1];
+  p.n2 = (int)n[2];
+  p.s0 = (int)stride[1];
+  p.s1 = (int)stride[2];
+  p.sq = stride[0];
+  const int bs = block_size_for(n[2]);
+  p.nb2 = (int)((n[2] + bs - 1) / bs);
+  const int64_t nb = (int64_t)p.nb2 * n[1] * (n[0] == 1 ? 1 : 2);
+  if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");
+  SlabHalo h{nullptr, nullptr, send_lo, send_hi};
+  g_err[0] = 0;
+  hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(bs), 0, (hipStream_t)stream, p, h);
+  return check_launch("lbm_halo_pack");
+}
+
+int lbm_slab_step(const lbm_step_args* a, int32_t part, const void* recv_lo, const void* recv_hi,
+                  void* send_lo, void* send_hi, void* stream) {
+  KParams p;
+  KernelFn fn;
+  dim3 grid, block;
+  int rc = prepare_step(a, p, fn, grid, block, true);
+  if (rc) return rc;
+  rc = check_slab_extent(a->stencil, a->n);
+  if (rc) return rc;
+  if (a->cell_class) return fail(LBM_E_UNSUPPORTED, "x-slabs run without boundaries (cell_class must be NULL)");
+  if (a->streaming != LBM_PRE_STREAMING)
+    return fail(LBM_E_UNSUPPORTED, "x-slabs use pull streaming (LBM_PRE_STREAMING)");
+  if (a->collision == LBM_COLL_LEARNED)
+    return fail(LBM_E_UNSUPPORTED, "x-slabs have no learned-collision kernel");
+  if (part < LBM_SLAB_ALL || part > LBM_SLAB_FACES) return fail(LBM_E_ARG, "unknown slab part %d", part);
+  const bool faces = part != LBM_SLAB_INTERIOR;
+  if (faces) {
+    if (!recv_lo || !recv_hi || !send_lo || !send_hi) return fail(LBM_E_ARG, "NULL halo pointer");
+    if (!distinct4(recv_lo, recv_hi, send_lo, send_hi))
+      return fail(LBM_E_ARG, "the four halo buffers must be distinct");
+  }
+  FaceFn ff = faces ? pick_face_kernel(a->stencil, a->storage, a->collision) : nullptr;
+  if (faces && !ff)
+    return fail(LBM_E_UNSUPPORTED, "no face kernel for stencil %d, storage %d, collision %d",
+                a->stencil, a->storage, a->collision);
+  g_err[0] = 0;
+  const int64_t n0 = a->n[0];
+  if (part != LBM_SLAB_FACES && n0 > 2) {
+    KParams pi = p;
+    pi.x0 = 1;
+    const int64_t nb = (int64_t)p.nb2 * a->n[1] * (n0 - 2);
+    pi.nblocks = (int)nb;
+    pi.order = block_order(pi, nb);
+    hipLaunchKernelGGL(fn, dim3((unsigned)nb), block, 0, (hipStream_t)stream, pi);
+  }
+  if (faces) {
+    const int64_t nb = (int64_t)p.nb2 * a->n[1] * (n0 == 1 ? 1 : 2);
+    SlabHalo h{recv_lo, recv_hi, send_lo, send_hi};
+    hipLaunchKernelGGL(ff, dim3((unsigned)nb), block, 0, (hipStream_t)stream, p, h);
+  }
+  return check_launch("lbm_slab_step");
 }
 
 }  // extern "C"

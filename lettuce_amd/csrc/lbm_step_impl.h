@@ -34,6 +34,7 @@ struct KParams {
   double tau_inv, trt_p, trt_m, beta, beta_inv;
   double rho0;
   double accel[3], guo;  // BGK_GUO
+  int x0 = 0;             // first i0 plane of the launch (slab interiors)
 };
 
 // Which row (i0, i1) and which run b2 of the row a block takes.  The
@@ -53,14 +54,14 @@ __device__ __forceinline__ void block_tile(const KParams& p, int b, int& i0, int
     b2 = j % p.nb2;
     const int t = j / p.nb2;
     i1 = k * h + t % h;
-    i0 = t / h;
+    i0 = p.x0 + t / h;
     return;
   }
   if (p.order == ORDER_XCD_SLAB) b = (b & 7) * (p.nblocks >> 3) + (b >> 3);
   b2 = b % p.nb2;
   const int r = b / p.nb2;
   i1 = r % p.n1;
-  i0 = r / p.n1;
+  i0 = p.x0 + r / p.n1;
 }
 
 template <class C>

@@ -6,3 +6,4 @@ from ._training import *  # noqa: F401,F403
 from ._flows import *  # noqa: F401,F403
 from ._reporter import *  # noqa: F401,F403
 from ._failure import *  # noqa: F401,F403
+from ._slab import *  # noqa: F401,F403

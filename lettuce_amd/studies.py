@@ -6,11 +6,13 @@ same case on the GPU.  Prints one JSON object per study.
     python -m lettuce_amd.studies cylinder     # D2Q9 channel + cylinder, KBC vs learned
     python -m lettuce_amd.studies turbulence3d # D3Q27 KBC decaying turbulence 384^3
     python -m lettuce_amd.studies train2d      # training of the learned collision
+    python -m lettuce_amd.studies slab         # one device's x-slab vs the periodic step
     python -m lettuce_amd.studies all
 
 Options: --steps N overrides a study's step count; --res N its resolution.
 """
 import argparse
+import ctypes
 import json
 import math
 import sys
@@ -371,10 +373,83 @@ def train2d(iters=300, n=256, fields=32, holdout=8, steps=16, every=4, u_rms=0.0
             "wall_s": round(time.perf_counter() - t_all, 2)}
 
 
+def slab(steps=100, stencil="D3Q27", collision="kbc", res=384, planes=48, storage="f32"):
+    """One device's share of a res^3 lattice split into x-slabs of `planes`
+    planes (48 x 384 x 384: 384^3 over 8 devices), initialised from the
+    Taylor-Green vortex.  The slab steps through lbm_slab_step with its
+    halos wrapped onto itself (a device copy per face stands in for the
+    transfer); after `steps` steps it must equal the periodic step of the
+    same array.  Times, with HIP events on the launch stream: the periodic
+    step, the slab step (interior + faces + the two copies) and each part."""
+    S = getattr(lt, stencil)()
+    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage)
+    flow = lt.TaylorGreenVortex(ctx, res, 1600, 0.1, S, initialize_fneq=False)
+    coll = lt.KBCCollision(flow.units.relaxation_parameter_lu) if collision == "kbc" else \
+        lt.BGKCollision(flow.units.relaxation_parameter_lu)
+    sim = lt.Simulation(flow, coll, [], lt.StreamingStrategy.PRE_STREAMING)
+    start = flow._state[:, :planes].clone()
+    sl = lt.SlabStep(sim, start.clone())
+    ref_a, ref_b = start.clone(), torch.empty_like(start)
+    args = N.LbmStepArgs.from_buffer_copy(sl.args)
+    stream = N.stream_handle(ctx.device)
+
+    def periodic(n):
+        # single fused steps of the slab as a periodic lattice
+        nonlocal ref_a, ref_b
+        args.f_in, args.f_out = ref_a.data_ptr(), ref_b.data_ptr()
+        for _ in range(n):
+            N.check(N.lib().lbm_step(ctypes.byref(args), stream), "lbm_step")
+            ref_a, ref_b = ref_b, ref_a
+            args.f_in, args.f_out = ref_a.data_ptr(), ref_b.data_ptr()
+
+    def slab_steps(n, part="all"):
+        for _ in range(n):
+            if part in ("all", "faces"):
+                sl.wrap()
+            sl.step(part)
+
+    periodic(steps)
+    sl.pack()
+    slab_steps(steps)
+    torch.cuda.synchronize()
+    diff = (sl.state.double() - ref_a.double()).abs().max().item()
+
+    def timed(fn, n=50):
+        fn(3)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn(n)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / n
+
+    def copies(n):
+        for _ in range(n):
+            sl.wrap()
+
+    t_per = timed(periodic)
+    t_all = timed(slab_steps)
+    t_int = timed(lambda n: slab_steps(n, "interior"))
+    t_face = timed(lambda n: [sl.step("faces") for _ in range(n)])
+    t_copy = timed(copies)
+    cells = planes * res * res
+    halo = sl.send_lo.numel() * sl.send_lo.element_size()
+    return {"study": "slab", "stencil": stencil, "collision": collision.upper(),
+            "storage": storage, "slab": [planes, res, res], "of_lattice": [res] * 3,
+            "devices": res // planes, "steps": steps,
+            "max_abs_diff_vs_periodic": diff,
+            "periodic_step_us": round(t_per, 2), "slab_step_us": round(t_all, 2),
+            "interior_us": round(t_int, 2), "faces_us": round(t_face, 2),
+            "halo_copies_us": round(t_copy, 2),
+            "mlups_periodic": round(cells / t_per, 1), "mlups_slab": round(cells / t_all, 1),
+            "halo_bytes_per_face": halo,
+            "xgmi_us_per_face_at_153GBs": round(halo / 153e9 * 1e6, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("study", choices=["tgv2d", "tgv3d", "cylinder", "turbulence3d", "train2d",
-                                      "all"])
+                                      "slab", "all"])
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--res", type=int, default=None)
@@ -398,6 +473,10 @@ def main():
     if args.study in ("turbulence3d", "all"):
         print(json.dumps(turbulence3d(**kw, **({"res": args.res} if args.res else {}))),
               flush=True)
+    if args.study in ("slab", "all"):
+        for stencil, coll, res, planes in (("D3Q27", "kbc", 384, 48), ("D3Q19", "bgk", 256, 32)):
+            print(json.dumps(slab(stencil=stencil, collision=coll,
+                                  res=args.res or res, planes=planes, **kw)), flush=True)
 
 
 if __name__ == "__main__":

@@ -106,6 +106,49 @@ def test_scan_rejects_bad_arguments_without_a_gpu():
                             None) == -1
 
 
+def test_slab_entry_points_reject_bad_arguments_without_a_gpu():
+    """x-slabs: 3D stencils only, pull streaming, no boundaries, no learned
+    collision, four distinct halo buffers; checked before any HIP call."""
+    lib = N.lib()
+    assert [lib.lbm_halo_planes(s) for s in (N.D2Q9, N.D3Q19, N.D3Q27)] == [-1, 5, 9]
+    a = N.LbmStepArgs()
+    a.abi_version = N.ABI_VERSION
+    a.stencil = N.D3Q19
+    a.collision = N.COLL_BGK
+    a.tau = 0.6
+    a.streaming = 2
+    a.n[:] = (4, 4, 4)
+    a.stride[:] = (64, 16, 4, 1)
+    a.f_in, a.f_out = 4096, 8192
+    h = [ctypes.c_void_p(v) for v in (1 << 16, 2 << 16, 3 << 16, 4 << 16)]
+    step = lib.lbm_slab_step
+    assert step(ctypes.byref(a), 3, *h, None) == -1               # unknown part
+    assert step(ctypes.byref(a), N.SLAB_ALL, h[0], h[0], h[2], h[3], None) == -1
+    assert b"distinct" in lib.lbm_last_error()
+    assert step(ctypes.byref(a), N.SLAB_FACES, None, h[1], h[2], h[3], None) == -1
+    a.streaming = 1
+    assert step(ctypes.byref(a), N.SLAB_ALL, *h, None) == -3      # push
+    a.streaming = 2
+    a.cell_class = 1 << 20
+    a.classes = 1 << 21
+    a.n_classes = 1
+    assert step(ctypes.byref(a), N.SLAB_ALL, *h, None) == -3      # boundaries
+    a.cell_class = a.classes = None
+    a.n_classes = 0
+    a.stencil = N.D2Q9
+    a.n[:] = (1, 4, 4)
+    a.stride[:] = (16, 16, 4, 1)
+    assert step(ctypes.byref(a), N.SLAB_ALL, *h, None) == -3      # 2D
+    n = (ctypes.c_int64 * 3)(4, 4, 4)
+    s = (ctypes.c_int64 * 4)(64, 16, 4, 1)
+    pack = lib.lbm_halo_pack
+    n2 = (ctypes.c_int64 * 3)(1, 4, 4)
+    s2 = (ctypes.c_int64 * 4)(16, 16, 4, 1)
+    assert pack(N.D2Q9, N.F32, n2, s2, h[0], h[1], h[2], None) == -3
+    assert pack(N.D3Q19, N.F32, n, s, h[0], h[1], h[1], None) == -1
+    assert pack(N.D3Q19, N.F32, n, s, None, h[1], h[2], None) == -1
+
+
 def test_training_rollout_needs_a_device_tensor():
     f = torch.zeros(9, 2, 4, 4)
     with pytest.raises(RuntimeError, match="ROCm GPU"):
