@@ -248,6 +248,7 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.fout = a->f_out;
   p.cls = a->cell_class;
   p.ct = a->classes;
+  p.nclasses = mask ? (int)a->n_classes : 0;
   p.bcf = a->bc_field;
   p.n0 = (int)a->n[0];
   p.n1 = (int)a->n[1];

@@ -76,11 +76,10 @@ __host__ __device__ constexpr int go(int q, int a) {
 // Storage codecs.  C is the arithmetic type.  Shifted formats store
 // f_q - w_q * rho0 so that the stored values are O(u) deviations.
 // ---------------------------------------------------------------------------
+// FP32 -> BF16 with round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+// (in place of five integer operations for the carry and the NaN check)
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);

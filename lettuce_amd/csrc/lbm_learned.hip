@@ -99,6 +99,7 @@ struct LearnedModel;
 
 template <>
 struct LearnedModel<D2Q9> {
+  static constexpr bool kMomentSpace = false;
   __device__ static void features(const float (&df)[9], float sc, float (&x)[6]) {
     const float xx = df[1] + df[3] + df[5] + df[6] + df[7] + df[8];
     const float yy = df[2] + df[4] + df[5] + df[6] + df[7] + df[8];
@@ -124,39 +125,27 @@ struct LearnedModel<D2Q9> {
   }
 };
 
-// D3Q27: the raw moments x^a y^b z^c (a, b, c in 0..2) are a tensor product
-// of the 1D moments (1, x, x^2) at x in {-1, 0, 1}, so moments and dual
-// vectors are three 1D transforms:
+// D3Q27 works in the basis of raw moments x^a y^b z^c (a, b, c in 0..2), a
+// tensor product of the 1D moments (1, x, x^2) at x in {-1, 0, 1}, so
+// moments and dual vectors are three 1D transforms:
 //   to moments   (v-, v0, v+) -> (v- + v0 + v+, v+ - v-, v+ + v-)
 //   back         (m0, m1, m2) -> ((m2 - m1) / 2, m0 - m2, (m1 + m2) / 2)
+// KBC's shear part ds carries exactly the second-order moments of df and
+// dh = df - ds the moments of order >= 3 (df has none of order 0 and 1), so
+//   f' = f - ds / tau - Phi(r . M dh) = f - Phi(w . M df),
+//   w = 1/tau on order 2, r_0 / r_1 / r_2 on orders 3 / 4 / 5-6, 0 below,
+// and the MLP inputs are second-order entries of the same M df.  M df is
+// M f (whose entries 000 and 100 / 010 / 001 are rho and j) minus the
+// equilibrium's raw moments in closed form: one forward and one backward
+// transform per cell, no per-population equilibrium, ds or dh.
 template <>
 struct LearnedModel<D3Q27> {
   using S = D3Q27;
-  __device__ static void features(const float (&df)[27], float sc, float (&x)[6]) {
-    float xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+  static constexpr bool kMomentSpace = true;
+  // t[a][b][c] <- sum_q e_x^a e_y^b e_z^c v_q
+  __device__ static void to_moments(const float (&v)[27], float (&t)[3][3][3]) {
 #pragma unroll
-    for (int q = 1; q < 27; ++q) {
-      const int ex = S::e[q][0], ey = S::e[q][1], ez = S::e[q][2];
-      if (ex != 0) xx += df[q];
-      if (ey != 0) yy += df[q];
-      if (ez != 0) zz += df[q];
-      if (ex * ey != 0) xy += (ex * ey > 0) ? df[q] : -df[q];
-      if (ex * ez != 0) xz += (ex * ez > 0) ? df[q] : -df[q];
-      if (ey * ez != 0) yz += (ey * ez > 0) ? df[q] : -df[q];
-    }
-    x[0] = (xx + yy + zz) * sc;
-    x[1] = (xx - zz) * sc;
-    x[2] = (yy - zz) * sc;
-    x[3] = xy * sc;
-    x[4] = xz * sc;
-    x[5] = yz * sc;
-  }
-  __device__ static void update(float (&f)[27], const float (&ds)[27], const float (&dh)[27],
-                                const float (&rate)[3], float tau_inv) {
-    // t[i][j][k]: i, j, k = velocity component + 1 along x, y, z
-    float t[3][3][3];
-#pragma unroll
-    for (int q = 0; q < 27; ++q) t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1] = dh[q];
+    for (int q = 0; q < 27; ++q) t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1] = v[q];
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -184,7 +173,60 @@ struct LearnedModel<D3Q27> {
         t[i][j][1] = c - a;
         t[i][j][2] = c + a;
       }
-    // t[a][b][c] = m_abc(dh); weight by the rate of its order group
+  }
+  // 1D moments sum_x W(x) x^p of the weights W(0) = 2/3, W(+-1) = 1/6, whose
+  // products are the D3Q27 weights; P3 = the raw moment abc of w
+  __host__ __device__ static constexpr float mu1(int p) {
+    return p == 0 ? 1.f : (p % 2 ? 0.f : 1.f / 3.f);
+  }
+  __host__ __device__ static constexpr float P3(int a, int b, int c) {
+    return mu1(a) * mu1(b) * mu1(c);
+  }
+  // t <- t - M feq(rho, u) for the quadratic equilibrium
+  // feq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u): each raw moment is
+  // rho (c0 P3(abc) + 3 sum_d u_d P3(abc + d) + 4.5 sum_dd' u_d u_d' P3(abc + d + d'))
+  __device__ static void subtract_equilibrium(float (&t)[3][3][3], float rho,
+                                              const float (&u)[3]) {
+    const float uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const float c0 = rho * (1.f - 1.5f * uu);
+    const float lx = 3.f * rho * u[0], ly = 3.f * rho * u[1], lz = 3.f * rho * u[2];
+    const float h = 4.5f * rho;
+    const float qxx = h * u[0] * u[0], qyy = h * u[1] * u[1], qzz = h * u[2] * u[2];
+    const float qxy = 2.f * h * u[0] * u[1], qxz = 2.f * h * u[0] * u[2],
+                qyz = 2.f * h * u[1] * u[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float m = 0.f;
+          if (P3(a, b, c) != 0.f) m += P3(a, b, c) * c0;
+          if (P3(a + 1, b, c) != 0.f) m += P3(a + 1, b, c) * lx;
+          if (P3(a, b + 1, c) != 0.f) m += P3(a, b + 1, c) * ly;
+          if (P3(a, b, c + 1) != 0.f) m += P3(a, b, c + 1) * lz;
+          if (P3(a + 2, b, c) != 0.f) m += P3(a + 2, b, c) * qxx;
+          if (P3(a, b + 2, c) != 0.f) m += P3(a, b + 2, c) * qyy;
+          if (P3(a, b, c + 2) != 0.f) m += P3(a, b, c + 2) * qzz;
+          if (P3(a + 1, b + 1, c) != 0.f) m += P3(a + 1, b + 1, c) * qxy;
+          if (P3(a + 1, b, c + 1) != 0.f) m += P3(a + 1, b, c + 1) * qxz;
+          if (P3(a, b + 1, c + 1) != 0.f) m += P3(a, b + 1, c + 1) * qyz;
+          t[a][b][c] -= m;
+        }
+  }
+  // T, Nxz, Nyz, Pxy, Pxz, Pyz of df from its raw moments
+  __device__ static void features_of(const float (&t)[3][3][3], float sc, float (&x)[6]) {
+    const float xx = t[2][0][0], yy = t[0][2][0], zz = t[0][0][2];
+    x[0] = (xx + yy + zz) * sc;
+    x[1] = (xx - zz) * sc;
+    x[2] = (yy - zz) * sc;
+    x[3] = t[1][1][0] * sc;
+    x[4] = t[1][0][1] * sc;
+    x[5] = t[0][1][1] * sc;
+  }
+  // f <- f - Phi(w . t)
+  __device__ static void relax(float (&f)[27], float (&t)[3][3][3], const float (&rate)[3],
+                               float tau_inv) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -192,7 +234,8 @@ struct LearnedModel<D3Q27> {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const int o = a + b + c;
-          t[a][b][c] *= o <= 2 ? 0.f : (o == 3 ? rate[0] : (o == 4 ? rate[1] : rate[2]));
+          t[a][b][c] *= o <= 1 ? 0.f
+                        : (o == 2 ? tau_inv : (o == 3 ? rate[0] : (o == 4 ? rate[1] : rate[2])));
         }
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -222,13 +265,57 @@ struct LearnedModel<D3Q27> {
         t[i][j][2] = 0.5f * (m1 + m2);
       }
 #pragma unroll
-    for (int q = 0; q < 27; ++q)
-      f[q] = f[q] - ds[q] * tau_inv - t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];
+    for (int q = 0; q < 27; ++q) f[q] -= t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];
   }
 };
 
+// gather_pull / gather_own with the loads of a cell grouped into rounds:
+// the class ids of the neighbours (boundaries only), then every population,
+// then the decodes.  Left alone, the scheduler interleaves loads with their
+// uses to save registers, which serialises the memory round trips of this
+// kernel.  `solid` is the SOLID flag of each class, staged in LDS.
+template <class S, class ST, bool MASK, bool PULL>
+__device__ __forceinline__ void gather_batched(const KParams& p, int cell, const int (&a0)[3],
+                                               const int (&a1)[3], const int (&a2)[3],
+                                               uint32_t nsm, uint32_t flags,
+                                               const uint8_t* solid,
+                                               typename ST::C (&f)[S::Q]) {
+  using T = typename ST::T;
+  const T* fin = (const T*)p.fin;
+  int src[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) src[q] = PULL ? nb_offset<S>(q, -1, a0, a1, a2) : cell;
+  // halfway bounce-back needs the class of every pull source of a
+  // near-solid fluid cell
+  const bool near = MASK && PULL &&
+                    (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
+  uint32_t cid[S::Q];
+  if (near) {
+#pragma unroll
+    for (int q = 1; q < S::Q; ++q) cid[q] = p.cls[src[q]];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  T raw[S::Q];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    int pop = q, idx = src[q];
+    if (MASK && PULL) {
+      if (near && q != 0 && solid[cid[q]]) {
+        pop = S::opp[q];  // the cell's own opposite population
+        idx = cell;
+      } else if ((nsm >> q) & 1u) {
+        idx = cell;
+      }
+    }
+    raw[q] = fin[pop * p.sq + idx];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+}
+
 template <class S, class ST, bool MASK, int MODE>
-__global__ void __launch_bounds__(256) learned_kernel(const KParams p, const LearnedArgs la) {
+__global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const LearnedArgs la) {
   using C = float;
   constexpr int Q = S::Q;
   constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
@@ -242,6 +329,31 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
   const CollParams<C> cp = coll_params<C>(p);
   const C tau_inv = (C)p.tau_inv;
   const bf16x8 zero8 = {};
+  // the biases (16 per lane and layer) in LDS as [layer][quarter][lane], so
+  // that each ds_read_b128 of a wave reads 1 KB without bank conflicts
+  __shared__ float4 bias[3][4][64];
+  for (int i = threadIdx.x; i < 3 * 4 * 64; i += blockDim.x) {
+    const int L = i / 256, c = (i / 64) % 4, l = i % 64;
+    const float* src = (L == 0 ? P->c1[l] : (L == 1 ? P->c2[l] : P->c3[l])) + 4 * c;
+    bias[L][c][l] = make_float4(src[0], src[1], src[2], src[3]);
+  }
+  __shared__ uint8_t solid[256];  // SOLID flag per cell class
+  if (MASK)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x)
+      solid[i] = i < p.nclasses ? (p.ct[i].flags & LBM_CLASS_SOLID) != 0 : 0;
+  __syncthreads();
+  auto ld_bias = [&](int L) {
+    f32x16 v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float4 b = bias[L][c][lane];
+      v[4 * c] = b.x;
+      v[4 * c + 1] = b.y;
+      v[4 * c + 2] = b.z;
+      v[4 * c + 3] = b.w;
+    }
+    return v;
+  };
 
   // the tile loop is uniform across the wave: every MFMA runs with all lanes
   for (int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < la.ntiles;
@@ -263,20 +375,31 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
       op = K.op;
     }
     C f[Q];
-    if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
-    else gather_own<S, ST>(p, cell, f);
+    gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, nsm, flags, solid, f);
 
     // non-equilibrium part and the MLP inputs
-    C rho, u[3], fe[Q], df[Q], ds[Q], dh[Q];
-    moments<S, C>(f, rho, u);
-    equilibrium<S, C>(rho, u, fe);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) df[q] = f[q] - fe[q];
-    kbc_shear<S, C>(df, ds);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) dh[q] = df[q] - ds[q];
+    using LM = LearnedModel<S>;
     C feat[6];
-    LearnedModel<S>::features(df, la.in_scale / rho, feat);
+    // D3Q27: the raw moments of df (mt); D2Q9: the KBC split (ds, dh)
+    C mt[LM::kMomentSpace ? 3 : 1][3][3], ds[LM::kMomentSpace ? 1 : Q], dh[LM::kMomentSpace ? 1 : Q];
+    if constexpr (LM::kMomentSpace) {
+      // rho and j are moments 000 and 100 / 010 / 001 of f
+      LM::to_moments(f, mt);
+      const C rho = mt[0][0][0], ri = C(1) / rho;
+      const C u[3] = {mt[1][0][0] * ri, mt[0][1][0] * ri, mt[0][0][1] * ri};
+      LM::subtract_equilibrium(mt, rho, u);
+      LM::features_of(mt, la.in_scale * ri, feat);
+    } else {
+      C rho, u[3], df[Q], fe[Q];
+      moments<S, C>(f, rho, u);
+      equilibrium<S, C>(rho, u, fe);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) df[q] = f[q] - fe[q];
+      kbc_shear<S, C>(df, ds);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) dh[q] = df[q] - ds[q];
+      LM::features(df, la.in_scale / rho, feat);
+    }
     bf16x8 x8 = zero8;
 #pragma unroll
     for (int k = 0; k < 6; ++k) x8[k] = (__bf16)feat[k];
@@ -286,14 +409,14 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
     // MLP, two 32-cell blocks per layer
     bf16x8 hA0, hA1, hB0, hB1;
     {
-      const f32x16 c1 = ld16(P->c1[lane]);
+      const f32x16 c1 = ld_bias(0);
       f32x16 dA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, bA, c1, 0, 0, 0);
       f32x16 dB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, bB, c1, 0, 0, 0);
       activate(dA, hA0, hA1);
       activate(dB, hB0, hB1);
     }
     {
-      const f32x16 c2 = ld16(P->c2[lane]);
+      const f32x16 c2 = ld_bias(1);
       f32x16 dA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A20, hA0, c2, 0, 0, 0);
       dA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A21, hA1, dA, 0, 0, 0);
       f32x16 dB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A20, hB0, c2, 0, 0, 0);
@@ -303,7 +426,7 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
     }
     C o[3];
     {
-      const f32x16 c3 = ld16(P->c3[lane]);
+      const f32x16 c3 = ld_bias(2);
       f32x16 dA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A30, hA0, c3, 0, 0, 0);
       dA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A31, hA1, dA, 0, 0, 0);
       f32x16 dB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A30, hB0, c3, 0, 0, 0);
@@ -324,8 +447,10 @@ __global__ void __launch_bounds__(256) learned_kernel(const KParams p, const Lea
       }
       if (fallback) {
         collide_kbc<S, C>(f, cp);
+      } else if constexpr (LM::kMomentSpace) {
+        LM::relax(f, mt, rate, tau_inv);
       } else {
-        LearnedModel<S>::update(f, ds, dh, rate, tau_inv);
+        LM::update(f, ds, dh, rate, tau_inv);
       }
     } else {
       apply_boundary_op<S, ST, PULL>(p, op, flags, cell, i0, i1, i2, f);

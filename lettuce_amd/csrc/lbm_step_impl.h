@@ -35,6 +35,7 @@ struct KParams {
   double rho0;
   double accel[3], guo;  // BGK_GUO
   int x0 = 0;             // first i0 plane of the launch (slab interiors)
+  int nclasses = 0;       // entries of ct (0 without boundaries)
 };
 
 // Which row (i0, i1) and which run b2 of the row a block takes.  The

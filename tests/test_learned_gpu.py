@@ -122,12 +122,17 @@ def test_d3q27_zero_mlp_is_bgk():
 
 
 def test_d3q27_unstable_rates_fall_back_to_kbc():
+    """every cell takes the KBC update: native KBC to the rounding of the
+    collision arithmetic, which the two kernels each compile (the learned
+    one may contract the same formulas into FMAs differently)"""
     p = zero_params()
     p.b3 = torch.full((3,), 5.0)
     flow = ArrayFlow(ctx(F32), _f27(seed=4), lt.D3Q27())
     out = lt.LearnedCollision(p)(flow).cpu().numpy()
     ref = lt.KBCCollision()(flow).cpu().numpy()
-    assert np.array_equal(out, ref)
+    err = rel_err(out, ref)
+    print(f"learned fallback vs KBC: {err:.3e}")
+    assert err <= 2e-6
 
 
 @pytest.mark.parametrize("seed", [0, 1])

@@ -149,6 +149,24 @@ def test_tgv3d_bgk_1000_steps_vs_oracle_fp32(name, n):
     assert rel_err(out, oflow.f.numpy()) <= REL32
 
 
+@pytest.mark.parametrize("layout", ["rows", "soa"])
+def test_tgv3d_two_step_kernel_1000_steps_vs_oracle_fp32(layout):
+    """The default D3Q19 path at 64^3 runs the two-step kernel (500
+    launches of two fused steps); after 1000 steps it meets the same 1e-5
+    bound against the oracle as the single-step kernel."""
+    s, os_ = _stencils()["D3Q19"]
+    oflow = O.TaylorGreenVortex(O.Context("cpu", F32), 64, 100, 0.1, os_)
+    f0 = oflow.f.numpy().copy()
+    tau = oflow.units.relaxation_parameter_lu
+    O.Simulation(oflow, O.BGKCollision(tau), [], O.StreamingStrategy.PRE_STREAMING)(1000)
+    c = lt.Context("cuda:0", dtype=F32, layout=layout)
+    flow = ArrayFlow(c, f0, s)
+    sim = lt.Simulation(flow, lt.BGKCollision(tau), [], lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(1000) == (500, 0)
+    sim(1000)
+    assert rel_err(flow.f.cpu().numpy(), oflow.f.numpy()) <= REL32
+
+
 def test_tgv3d_golden_10_steps():
     g = golden("g2_tgv3d_bgk.npz")
     for name in ("D3Q19", "D3Q27"):
@@ -321,6 +339,29 @@ def test_halfway_channel_cylinder_vs_oracle(dtype, strategy):
 
 
 # ---------------------------------------------------------- shifted storage
+@pytest.mark.parametrize("storage,tdt", [("f16_shifted", torch.float16),
+                                         ("bf16_shifted", torch.bfloat16)])
+def test_16_bit_encoding_rounds_to_nearest_even(storage, tdt):
+    """lbm_encode with rho0 = 0 is the plain FP32 -> 16-bit conversion: bit
+    for bit torch's round to nearest even, on random values, exact ties
+    (a 16-bit value plus half its unit in the last place, both parities),
+    signed zeros and FP32 subnormals."""
+    g = torch.Generator().manual_seed(4)
+    v = (torch.randn(9, 32, 32, generator=g) * 10.0 ** torch.randint(-6, 2, (9, 32, 32),
+                                                                      generator=g)).to(F32)
+    mant = 8 if tdt == torch.bfloat16 else 11
+    base = v[:4].to(tdt).to(F32)
+    half = torch.where(base == 0, torch.zeros_like(base),
+                       2.0 ** (torch.floor(torch.log2(base.abs())) - mant))
+    v[:4] = base + half                                   # ties
+    v[4, 0, :6] = torch.tensor([0.0, -0.0, 1e-39, -3e-40, 1.4e-45, -1e-38])
+    f = torch.empty(9, 32, 32, dtype=tdt, device="cuda:0")
+    lt._ops.encode(lt.D2Q9(), v, f, 0.0)
+    torch.cuda.synchronize()
+    ref = v.to(tdt)
+    assert torch.equal(f.cpu().view(torch.int16), ref.view(torch.int16))
+
+
 @pytest.mark.parametrize("storage,tdt", [("f16_shifted", torch.float16),
                                          ("bf16_shifted", torch.bfloat16)])
 def test_shifted_storage_one_step_rounding(storage, tdt):
