@@ -269,51 +269,6 @@ struct LearnedModel<D3Q27> {
   }
 };
 
-// gather_pull / gather_own with the loads of a cell grouped into rounds:
-// the class ids of the neighbours (boundaries only), then every population,
-// then the decodes.  Left alone, the scheduler interleaves loads with their
-// uses to save registers, which serialises the memory round trips of this
-// kernel.  `solid` is the SOLID flag of each class, staged in LDS.
-template <class S, class ST, bool MASK, bool PULL>
-__device__ __forceinline__ void gather_batched(const KParams& p, int cell, const int (&a0)[3],
-                                               const int (&a1)[3], const int (&a2)[3],
-                                               uint32_t nsm, uint32_t flags,
-                                               const uint8_t* solid,
-                                               typename ST::C (&f)[S::Q]) {
-  using T = typename ST::T;
-  const T* fin = (const T*)p.fin;
-  int src[S::Q];
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) src[q] = PULL ? nb_offset<S>(q, -1, a0, a1, a2) : cell;
-  // halfway bounce-back needs the class of every pull source of a
-  // near-solid fluid cell
-  const bool near = MASK && PULL &&
-                    (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
-  uint32_t cid[S::Q];
-  if (near) {
-#pragma unroll
-    for (int q = 1; q < S::Q; ++q) cid[q] = p.cls[src[q]];
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  T raw[S::Q];
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    int pop = q, idx = src[q];
-    if (MASK && PULL) {
-      if (near && q != 0 && solid[cid[q]]) {
-        pop = S::opp[q];  // the cell's own opposite population
-        idx = cell;
-      } else if ((nsm >> q) & 1u) {
-        idx = cell;
-      }
-    }
-    raw[q] = fin[pop * p.sq + idx];
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
-}
-
 template <class S, class ST, bool MASK, int MODE>
 __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const LearnedArgs la) {
   using C = float;
@@ -338,9 +293,7 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
     bias[L][c][l] = make_float4(src[0], src[1], src[2], src[3]);
   }
   __shared__ uint8_t solid[256];  // SOLID flag per cell class
-  if (MASK)
-    for (int i = threadIdx.x; i < 256; i += blockDim.x)
-      solid[i] = i < p.nclasses ? (p.ct[i].flags & LBM_CLASS_SOLID) != 0 : 0;
+  if (MASK) stage_solid_flags(p, solid);
   __syncthreads();
   auto ld_bias = [&](int L) {
     f32x16 v;
