@@ -25,7 +25,10 @@
 // layer duplicated in both halves, and the last layer places its three
 // outputs in rows 0-2 and 4-6, so every lane ends up holding the rates of
 // its own cell.
+#include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
 
 #include "lbm_step_impl.h"
 
@@ -459,15 +462,29 @@ int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int stor
   la.stable_hi = l->stable_hi;
   la.nt2 = p.n2 / 64;
   la.ntiles = p.n0 * p.n1 * la.nt2;
+  // Blocks: about three 64-cell tiles per wave (at most 4096 blocks), but
+  // at least two rounds of blocks at the kernel's occupancy.  Measured on
+  // MI355X: the 2048 x 512 D2Q9 channel (16384 tiles) runs 16 % faster with
+  // ~2.7 tiles per wave than with one (the per-block set-up of weights,
+  // biases and class flags is amortised), D3Q27 256^3 (262144 tiles) 3 %
+  // faster with 4096 blocks than with one round.
   const int waves_per_block = 4;
-  int blocks = (la.ntiles + waves_per_block - 1) / waves_per_block;
-  if (blocks > 4096) blocks = 4096;
+  const int64_t all = (la.ntiles + waves_per_block - 1) / waves_per_block;
+  int64_t blocks = std::min<int64_t>(4096, (la.ntiles + 3 * waves_per_block - 1) /
+                                               (3 * waves_per_block));
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 64 * waves_per_block,
+                                                   0) == hipSuccess)
+    blocks = std::max<int64_t>(blocks, std::min<int64_t>(all, 2 * (int64_t)per_cu * cus));
+  blocks = std::max<int64_t>(1, std::min(blocks, all));
   KParams q = p;
   const void* bufs[2] = {f_in, f_out};
   for (int64_t k = 0; k < nsteps; ++k) {
     q.fin = bufs[k & 1];
     q.fout = (void*)bufs[(k + 1) & 1];
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(64 * waves_per_block), 0, stream, q, la);
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64 * waves_per_block), 0, stream, q, la);
   }
   return LBM_OK;
 }
