@@ -141,6 +141,10 @@ struct StBF16 {
 // (lettuce/ext/_equilibrium/quadratic_equilibrium.py:11-24 with cs^2 = 1/3:
 //  feq = w rho ((2 e.u - u.u) / (2 cs^2) + (e.u / cs^2)^2 / 2 + 1)).
 // ---------------------------------------------------------------------------
+// 1 / x: v_rcp_f32 (1 ulp) in FP32, a correctly rounded division in FP64
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 template <class S, class C>
 __device__ __forceinline__ void moments(const C (&f)[S::Q], C& rho, C (&u)[3]) {
   C r = 0, jx = 0, jy = 0, jz = 0;
@@ -152,7 +156,7 @@ __device__ __forceinline__ void moments(const C (&f)[S::Q], C& rho, C (&u)[3]) {
     if (S::D == 3 && S::e[q][2] != 0) jz += S::e[q][2] > 0 ? f[q] : -f[q];
   }
   rho = r;
-  const C ri = C(1) / r;
+  const C ri = recip(r);
   u[0] = jx * ri;
   u[1] = jy * ri;
   u[2] = S::D == 3 ? jz * ri : C(0);
@@ -173,11 +177,28 @@ __device__ __forceinline__ C feq_q(int q, C rho, const C (&u)[3], C uu) {
   return C(S::w[q]) * rho * (C(1.5) * (C(2) * eu - uu) + C(4.5) * eu * eu + C(1));
 }
 
+// all Q values at once: opposite populations share the even part
+// w rho (c + 4.5 (e.u)^2), c = 1 - 1.5 u.u, and differ in the sign of the
+// odd part 3 w rho (e.u)
 template <class S, class C>
 __device__ __forceinline__ void equilibrium(C rho, const C (&u)[3], C (&feq)[S::Q]) {
   const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const C c = C(1) - C(1.5) * uu;
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) feq[q] = feq_q<S, C>(q, rho, u, uu);
+  for (int q = 0; q < S::Q; ++q) {
+    const int o = S::opp[q];
+    if (o < q) continue;
+    const C wr = C(S::w[q]) * rho;
+    if (o == q) {
+      feq[q] = wr * c;
+      continue;
+    }
+    const C eu = eu_of<S, C>(q, u);
+    const C even = wr * (c + C(4.5) * eu * eu);
+    const C odd = C(3) * wr * eu;
+    feq[q] = even + odd;
+    feq[o] = even - odd;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -194,16 +215,34 @@ struct CollParams {
   C guo;          // BGK_GUO: 1 - 1 / (2 force_tau)
 };
 
-// BGK: f - (f - feq) / tau  (lettuce/ext/_collision/bgk_collision.py:17-22)
+// BGK: f - (f - feq) / tau  (lettuce/ext/_collision/bgk_collision.py:17-22),
+// evaluated as (1 - w) f + w feq with w = 1 / tau and, for each pair of
+// opposite populations (e.u of one is -e.u of the other),
+//   w feq = w_q rho w (c + 4.5 (e.u)^2) +- 3 w_q rho w (e.u),  c = 1 - 1.5 u.u
+// so the even and odd parts of the equilibrium are formed once per pair.
 template <class S, class C>
 __device__ __forceinline__ void collide_bgk(C (&f)[S::Q], const CollParams<C>& cp) {
   C rho, u[3];
   moments<S, C>(f, rho, u);
   const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const C om = cp.tau_inv, keep = C(1) - om;
+  const C a = om * rho;
+  const C c = C(1) - C(1.5) * uu;
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
-    const C fe = feq_q<S, C>(q, rho, u, uu);
-    f[q] = f[q] - cp.tau_inv * (f[q] - fe);
+    const int o = S::opp[q];
+    if (o < q) continue;  // the pair was relaxed at its first index
+    const C wa = C(S::w[q]) * a;
+    if (o == q) {
+      f[q] = keep * f[q] + wa * c;
+      continue;
+    }
+    const C eu = eu_of<S, C>(q, u);
+    const C even = wa * (c + C(4.5) * eu * eu);
+    const C odd = C(3) * wa * eu;
+    const C fq = f[q], fo = f[o];
+    f[q] = keep * fq + (even + odd);
+    f[o] = keep * fo + (even - odd);
   }
 }
 
@@ -325,7 +364,7 @@ __device__ __forceinline__ C kbc_gamma(const C (&ds)[S::Q], const C (&dh)[S::Q],
   C sum_s = 0, sum_h = 0;
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
-    const C r = dh[q] / fe[q];
+    const C r = dh[q] * recip(fe[q]);
     sum_s += ds[q] * r;
     sum_h += dh[q] * r;
   }
