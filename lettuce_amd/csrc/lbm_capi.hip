@@ -157,9 +157,9 @@ static int block_order(const KParams& p, int64_t nb) {
 // the two-step kernel for this step, if one applies: periodic D3Q19 or
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
 // TB_Y.  Used by default for D3Q19, with one barrier per plane (measured on
-// MI355X, BGK TGV 256^3: FP32 38.8k -> 58.0k MLUPS, FP16-shifted 79.0k ->
-// 83.7k; with two barriers per plane FP16 fell to 78.4k).  D3Q27 KBC loses
-// (27.1k -> 25.1k: 94 KB of LDS and 140 VGPRs leave one block per CU).
+// MI355X, BGK TGV 256^3: FP32 38.6k -> 58.9k MLUPS, FP16-shifted 92.8k; with
+// two barriers per plane FP16 was 5 % slower).  D3Q27 KBC loses (27.1k ->
+// 25.1k: 125-143 VGPRs leave one 11-wavefront block per CU).
 // LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it applies;
 // LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64),
 // LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=2
