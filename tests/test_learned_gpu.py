@@ -158,3 +158,23 @@ def test_d3q27_learned_30_steps_vs_oracle(storage):
     lt.Simulation(flow, lt.LearnedCollision(p), [], lt.StreamingStrategy.PRE_STREAMING)(30)
     err = rel_err(flow.f.cpu().numpy(), oflow.f.numpy())
     assert err <= (1e-5 if storage == "f32" else 2e-2), err
+
+
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+def test_d3q27_learned_with_bounce_back_vs_oracle(strategy):
+    """the masked learned kernel (D3Q27 works in raw-moment space): a
+    fullway bounce-back sphere in a periodic box, 20 steps against the
+    oracle's learned collision with lettuce's BounceBackBoundary"""
+    p, op = pair(1, out_scale=0.1)
+    res = [8, 16, 64]
+    f0 = _f27(res=res, seed=12, amp=0.01)
+    x, y, z = torch.meshgrid(*[torch.arange(n, dtype=F64) for n in res], indexing="ij")
+    mask = (x - 3.5) ** 2 + (y - 7.5) ** 2 + (z - 20.0) ** 2 < 3.2 ** 2
+    oflow = _Grid(O.D3Q27(), res, F32, f0)
+    oflow.post_boundaries = [O.BounceBackBoundary(mask)]
+    O.Simulation(oflow, O.LearnedCollision(op), [],
+                 getattr(O.StreamingStrategy, strategy))(20)
+    flow = ArrayFlow(ctx(F32), f0, lt.D3Q27(), post=[lt.BounceBackBoundary(mask)])
+    lt.Simulation(flow, lt.LearnedCollision(p), [], getattr(lt.StreamingStrategy, strategy))(20)
+    err = rel_err(flow.f.cpu().numpy(), oflow.f.numpy())
+    assert err <= 1e-5, err
