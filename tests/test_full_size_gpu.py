@@ -1,0 +1,130 @@
+"""Checks at the bench workloads' full sizes, through properties that do not
+need a CPU reference of that size: the two-step kernel against single steps
+on the 256^3 headline grid, conservation of mass and momentum (FP64
+reductions), the shifted storage's decode -> encode round trip, the learned
+collision in BF16 against FP32, and an x-slab of one of 8 devices at its real
+shape against the periodic step of the same array."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import lettuce_amd as lt
+from lettuce_amd import _native as N
+from lettuce_amd import _ops
+
+pytestmark = pytest.mark.gpu
+
+F32 = torch.float32
+
+
+def _tgv(stencil, n, storage="f32", layout="rows"):
+    ctx = lt.Context("cuda:0", dtype=F32, storage=storage, layout=layout)
+    return lt.TaylorGreenVortex(ctx, n, 1600, 0.1, stencil, initialize_fneq=False)
+
+
+def _sums(flow):
+    return _ops.reduce_f64(flow.stencil, flow._state, flow.rho0_shift).cpu().numpy()
+
+
+def test_headline_two_step_matches_single_steps_at_256(monkeypatch):
+    """D3Q19 BGK TGV 256^3 FP32 (the bench workload), 20 steps: the default
+    path (10 two-step launches) against 20 single-step launches, and mass
+    and momentum conserved"""
+    S = lt.D3Q19()
+    runs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("LETTUCE_AMD_TB2", mode)
+        flow = _tgv(S, 256)
+        sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                            lt.StreamingStrategy.PRE_STREAMING)
+        assert sim._collide_and_stream.plan(20) == ((10, 0) if mode == "1" else (0, 20))
+        s0 = _sums(flow)
+        sim(20)
+        s1 = _sums(flow)
+        # mass and momentum to FP32 rounding of 16.8 M cells
+        assert abs(s1[0] - s0[0]) <= 1e-6 * s0[0], (s0[0], s1[0])
+        assert np.all(np.abs(s1[1:4] - s0[1:4]) <= 1e-9 * s0[0]), (s0[1:4], s1[1:4])
+        runs[mode] = flow._state.clone()
+        del sim, flow
+    # the collision arithmetic may round apart by a few ulp per step
+    diff = (runs["1"] - runs["0"]).abs().max().item()
+    assert diff <= 2e-6, diff
+
+
+def test_d3q27_kbc_conserves_mass_and_momentum_at_256():
+    S = lt.D3Q27()
+    flow = _tgv(S, 256)
+    sim = lt.Simulation(flow, lt.KBCCollision(flow.units.relaxation_parameter_lu), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    s0 = _sums(flow)
+    sim(10)
+    s1 = _sums(flow)
+    assert abs(s1[0] - s0[0]) <= 1e-6 * s0[0]
+    assert np.all(np.abs(s1[1:4] - s0[1:4]) <= 1e-9 * s0[0]), (s0[1:4], s1[1:4])
+    scan = _ops.scan_f64(S, flow._state, flow.rho0_shift).cpu().numpy()
+    assert scan[0] == 0 and scan[1] == 0
+
+
+@pytest.mark.parametrize("storage", ["f16_shifted", "bf16_shifted"])
+def test_shifted_storage_round_trip_is_idempotent_at_256(storage):
+    """decode -> encode of a stepped 256^3 D3Q19 state returns the same
+    stored values (bit for bit but for the sign of zero: a stored -0 decodes
+    to w rho0 and encodes back as +0)"""
+    S = lt.D3Q19()
+    flow = _tgv(S, 256, storage)
+    sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    sim(4)
+    raw = flow._state.clone()
+    decoded = _ops.decode(S, flow._state, flow.rho0_shift)
+    again = torch.empty_like(flow._state)
+    _ops.encode(S, decoded, again, flow.rho0_shift)
+    assert torch.equal(again, raw)
+    nonzero = raw != 0
+    assert torch.equal(again.view(torch.int16)[nonzero], raw.view(torch.int16)[nonzero])
+
+
+def test_learned_bf16_tracks_fp32_at_256():
+    """the C5 pairing at 256^3: learned collision with shifted BF16 storage
+    against FP32 storage, 10 steps from the same TGV; mass conserved to the
+    16-bit rounding"""
+    S = lt.D3Q27()
+    params = lt.LearnedParameters.seeded(0, out_scale=0.1)
+    out = {}
+    for storage in ("f32", "bf16_shifted"):
+        flow = _tgv(S, 256, storage)
+        sim = lt.Simulation(flow, lt.LearnedCollision(params), [],
+                            lt.StreamingStrategy.PRE_STREAMING)
+        s0 = _sums(flow)
+        sim(10)
+        s1 = _sums(flow)
+        assert abs(s1[0] - s0[0]) <= (1e-6 if storage == "f32" else 1e-4) * s0[0]
+        out[storage] = flow.f.to(F32)
+        del sim, flow
+    err = ((out["bf16_shifted"] - out["f32"]).abs().max() / out["f32"].abs().max()).item()
+    assert err <= 2e-2, err
+
+
+def test_slab_of_eight_devices_wrapped_is_the_periodic_step():
+    """a 48 x 384 x 384 D3Q27 KBC slab (384^3 over 8 devices) whose halos are
+    its own: 10 steps bit for bit equal to the periodic step of the array"""
+    S = lt.D3Q27()
+    flow = _tgv(S, 384, layout="soa")
+    sim = lt.Simulation(flow, lt.KBCCollision(flow.units.relaxation_parameter_lu), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    start = flow._state[:, :48].clone()
+    slab = lt.SlabStep(sim, start.clone())
+    args = N.LbmStepArgs.from_buffer_copy(slab.args)
+    a, b = start.clone(), torch.empty_like(start)
+    for _ in range(10):
+        args.f_in, args.f_out = a.data_ptr(), b.data_ptr()
+        N.check(N.lib().lbm_step(ctypes.byref(args), N.stream_handle(a.device)), "lbm_step")
+        a, b = b, a
+    slab.pack()
+    for _ in range(10):
+        slab.wrap()
+        slab.step()
+    torch.cuda.synchronize()
+    assert torch.equal(slab.state, a)
