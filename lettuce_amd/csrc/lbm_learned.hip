@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
     if constexpr (LM::kMomentSpace) {
       // rho and j are moments 000 and 100 / 010 / 001 of f
       LM::to_moments(f, mt);
-      const C rho = mt[0][0][0], ri = C(1) / rho;
+      const C rho = mt[0][0][0], ri = recip(rho);
       const C u[3] = {mt[1][0][0] * ri, mt[0][1][0] * ri, mt[0][0][1] * ri};
       LM::subtract_equilibrium(mt, rho, u);
       LM::features_of(mt, la.in_scale * ri, feat);
