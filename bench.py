@@ -192,8 +192,9 @@ def main():
                          "'rows' for periodic flows, 'soa' for flows with boundaries")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the slab decomposition over xGMI is described in DESIGN.md); "
+        print("bench.py measures one GPU: the library steps one device's x-slab "
+              "(lettuce_amd.SlabStep) and leaves the halo transfer between "
+              "devices to the caller, so there is no multi-GPU bench; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
