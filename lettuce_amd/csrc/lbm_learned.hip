@@ -295,8 +295,8 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
     const float* src = (L == 0 ? P->c1[l] : (L == 1 ? P->c2[l] : P->c3[l])) + 4 * c;
     bias[L][c][l] = make_float4(src[0], src[1], src[2], src[3]);
   }
-  __shared__ uint8_t solid[256];  // SOLID flag per cell class
-  if (MASK) stage_solid_flags(p, solid);
+  __shared__ lbm_cell_class ctl[MASK ? 256 : 1];  // the cell-class table
+  if (MASK) stage_classes(p, ctl);
   __syncthreads();
   auto ld_bias = [&](int L) {
     f32x16 v;
@@ -323,15 +323,9 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
     axis_offsets(i1, p.n1, p.s1, a1);
     axis_offsets(i2, p.n2, 1, a2);
     const int cell = a0[1] + a1[1] + a2[1];
-    uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
-    if (MASK) {
-      const lbm_cell_class& K = p.ct[p.cls[cell]];
-      nsm = K.no_stream;
-      flags = K.flags;
-      op = K.op;
-    }
+    uint32_t nsm, flags, op;
     C f[Q];
-    gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, nsm, flags, solid, f);
+    gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
 
     // non-equilibrium part and the MLP inputs
     using LM = LearnedModel<S>;

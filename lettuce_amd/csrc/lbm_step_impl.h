@@ -136,45 +136,60 @@ __device__ __forceinline__ void gather_pull(const KParams& p, int cell,
   }
 }
 
-// solid[c] = SOLID flag of cell class c (0 past the table), staged in LDS
-// by the whole block before its first gather_batched
-__device__ __forceinline__ void stage_solid_flags(const KParams& p, uint8_t* solid) {
-  for (int i = threadIdx.x; i < 256; i += blockDim.x)
-    solid[i] = i < p.nclasses ? (p.ct[i].flags & LBM_CLASS_SOLID) != 0 : 0;
+// The first p.nclasses entries of the cell-class table, copied into LDS by
+// the whole block before its first gather_batched (entries past them are
+// never read for a valid class map; the array covers every uint8 id).
+__device__ __forceinline__ void stage_classes(const KParams& p, lbm_cell_class* ctl) {
+  const uint32_t* src = (const uint32_t*)p.ct;
+  uint32_t* dst = (uint32_t*)ctl;
+  const int words = p.nclasses * (int)(sizeof(lbm_cell_class) / 4);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
 // gather_pull / gather_own with the loads of a cell grouped into rounds:
-// the class ids of the neighbours (boundaries only), then every population,
-// then the decodes.  Left alone, the scheduler interleaves loads with their
-// uses to save registers, which serialises the memory round trips of this
-// kernel.  `solid` is the SOLID flag of each class, staged in LDS.
+// with boundaries, first the class ids of the cell and of all its pull
+// sources (uint8 reads of the class map; their fields come from the LDS
+// table ctl), then every population, then the decodes.  Left alone, the
+// scheduler interleaves loads with their uses to save registers, which
+// serialises the memory round trips of these latency-bound kernels.
+// Returns the cell's no-stream bits, flags and op in nsm, flags, op.
 template <class S, class ST, bool MASK, bool PULL>
 __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const int (&a0)[3],
                                                const int (&a1)[3], const int (&a2)[3],
-                                               uint32_t nsm, uint32_t flags,
-                                               const uint8_t* solid,
+                                               const lbm_cell_class* ctl, uint32_t& nsm,
+                                               uint32_t& flags, uint32_t& op,
                                                typename ST::C (&f)[S::Q]) {
   using T = typename ST::T;
   const T* fin = (const T*)p.fin;
   int src[S::Q];
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) src[q] = PULL ? nb_offset<S>(q, -1, a0, a1, a2) : cell;
-  // halfway bounce-back needs the class of every pull source of a
-  // near-solid fluid cell
-  const bool near = MASK && PULL &&
-                    (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
-  uint32_t cid[S::Q];
-  if (near) {
+  uint32_t c0 = 0, cid[S::Q];
+  if (MASK) {
+    c0 = p.cls[cell];
+    if (PULL) {
 #pragma unroll
-    for (int q = 1; q < S::Q; ++q) cid[q] = p.cls[src[q]];
+      for (int q = 1; q < S::Q; ++q) cid[q] = p.cls[src[q]];
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
+  nsm = 0;
+  flags = 0;
+  op = LBM_OP_COLLIDE;
+  if (MASK) {
+    nsm = ctl[c0].no_stream;
+    flags = ctl[c0].flags;
+    op = ctl[c0].op;
+  }
+  // halfway bounce-back: a near-solid fluid cell whose pull source is solid
+  const bool near = MASK && PULL &&
+                    (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
   T raw[S::Q];
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
     int pop = q, idx = src[q];
     if (MASK && PULL) {
-      if (near && q != 0 && solid[cid[q]]) {
+      if (near && q != 0 && (ctl[cid[q]].flags & LBM_CLASS_SOLID)) {
         pop = S::opp[q];  // the cell's own opposite population
         idx = cell;
       } else if ((nsm >> q) & 1u) {
@@ -345,10 +360,10 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
   constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
   constexpr bool PUSH = (MODE & LBM_POST_STREAMING) != 0;
 
-  // with boundaries, the SOLID flag of every class (for halfway bounce-back)
-  __shared__ uint8_t solid[MASK ? 256 : 1];
+  // with boundaries, the cell-class table in LDS
+  __shared__ lbm_cell_class ctl[MASK ? 256 : 1];
   if (MASK) {
-    stage_solid_flags(p, solid);
+    stage_classes(p, ctl);
     __syncthreads();
   }
 
@@ -364,17 +379,14 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
   const int cell = a0[1] + a1[1] + a2[1];
 
   uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
-  if (MASK) {
-    const lbm_cell_class& K = p.ct[p.cls[cell]];
-    nsm = K.no_stream;
-    flags = K.flags;
-    op = K.op;
-  }
-
   C f[S::Q];
-  if (MASK && PULL) gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, nsm, flags, solid, f);
-  else if (PULL) gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
-  else gather_own<S, ST>(p, cell, f);
+  if (MASK) {
+    gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
+  } else if (PULL) {
+    gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
+  } else {
+    gather_own<S, ST>(p, cell, f);
+  }
 
   const CollParams<C> cp = coll_params<C>(p);
   if (!MASK || op == LBM_OP_COLLIDE) collide<S, C, COLL>(f, cp);
