@@ -5,6 +5,7 @@ reductions), the shifted storage's decode -> encode round trip, the learned
 collision in BF16 against FP32, and an x-slab of one of 8 devices at its real
 shape against the periodic step of the same array."""
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -128,3 +129,60 @@ def test_slab_of_eight_devices_wrapped_is_the_periodic_step():
         slab.step()
     torch.cuda.synchronize()
     assert torch.equal(slab.state, a)
+
+
+class _TiledFlow(lt.ExtFlow):
+    """D3Q19 at n^3 whose initial populations repeat with period h along x
+    and y: a TGV-like equilibrium built on one h x h x n block and copied
+    into the other three, so the copies are bit for bit equal"""
+
+    def __init__(self, context, n, h):
+        self._n, self._h = n, h
+        super().__init__(context, [n, n, n], 100.0, 0.05, lt.D3Q19())
+
+    def make_resolution(self, resolution, stencil=None):
+        return list(resolution)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return lt.UnitConversion(reynolds_number, mach_number,
+                                 characteristic_length_lu=resolution[0])
+
+    def initial_pu(self):
+        raise NotImplementedError
+
+    def initialize(self):
+        n, h, f = self._n, self._h, self._state
+        dev = f.device
+        a = torch.arange(h, device=dev, dtype=F32) * (2 * math.pi / h)
+        c = torch.cos(torch.arange(n, device=dev, dtype=F32) * (2 * math.pi / n))
+        u = torch.zeros(3, h, h, n, device=dev, dtype=F32)
+        u[0] = 0.05 * torch.sin(a)[:, None, None] * torch.cos(a)[None, :, None] * c
+        u[1] = -0.05 * torch.cos(a)[:, None, None] * torch.sin(a)[None, :, None] * c
+        rho = torch.ones(h, h, n, device=dev, dtype=F32)
+        _ops.equilibrium(self.stencil, f[:, :h, :h], rho, u)
+        del u, rho
+        for i, j in ((1, 0), (0, 1), (1, 1)):
+            f[:, i * h:(i + 1) * h, j * h:(j + 1) * h].copy_(f[:, :h, :h])
+
+
+def test_largest_d3q19_grid_keeps_its_translation_symmetry():
+    """D3Q19 BGK FP32 at 1024^3 (1.07 G cells, two 76 GB population
+    arrays): a state that repeats with period 512 along x and y keeps doing
+    so bit for bit through 8 steps of the two-step kernel, since every cell
+    runs the same arithmetic on the same neighbourhood.  This checks the
+    32-bit cell offsets near 2^30 and the tiling at the largest grid one
+    device holds; mass is conserved as well."""
+    n, h = 1024, 512
+    # SoA: the `rows` layout interleaves the populations of a row, so its
+    # cell offsets span all 19 arrays and exceed 32 bits at this size
+    flow = _TiledFlow(lt.Context("cuda:0", dtype=F32, layout="soa"), n, h)
+    sim = lt.Simulation(flow, lt.BGKCollision(0.51), [], lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (4, 0)
+    s0 = _sums(flow)
+    sim(8)
+    s1 = _sums(flow)
+    assert abs(s1[0] - s0[0]) <= 1e-6 * s0[0], (s0[0], s1[0])
+    f = flow._state
+    assert torch.isfinite(f[:, :h, :h]).all()
+    for i, j in ((1, 0), (0, 1), (1, 1)):
+        assert torch.equal(f[:, i * h:(i + 1) * h, j * h:(j + 1) * h], f[:, :h, :h]), (i, j)
