@@ -85,6 +85,27 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// Element i of a population array p (a pointer uniform across the wave).
+// Element offsets are below 2^31 (checked on the host), so for 16-bit
+// formats the byte offset 2 i fits 32 bits: p stays in scalar registers
+// and each access takes one 32-bit vector offset (the global_load saddr
+// form) instead of a 64-bit address computed per lane.  Wider formats keep
+// 64-bit addresses (4 i may exceed 2^32).
+template <class T>
+__device__ __forceinline__ const T& elem(const T* p, uint32_t i) {
+  if constexpr (sizeof(T) == 2)
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i << 1));
+  else
+    return p[i];
+}
+template <class T>
+__device__ __forceinline__ T* elem_ptr(T* p, uint32_t i) {
+  if constexpr (sizeof(T) == 2)
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (i << 1));
+  else
+    return p + i;
+}
+
 // dec: stored value -> population, enc: population -> stored value;
 // ld / st: one element, stores bypassing the caches (streamed output)
 struct StF32 {
@@ -116,10 +137,10 @@ struct StF16 {
   __device__ static C dec(T x, C sh) { return (float)x + sh; }
   __device__ static T enc(C v, C sh) { return (_Float16)(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
-    return (float)p[i] + sh;
+    return (float)elem(p, i) + sh;
   }
   __device__ static void st(T* p, uint32_t i, C v, C sh) {
-    __builtin_nontemporal_store((_Float16)(v - sh), p + i);
+    __builtin_nontemporal_store((_Float16)(v - sh), elem_ptr(p, i));
   }
 };
 struct StBF16 {
@@ -129,10 +150,10 @@ struct StBF16 {
   __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
   __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
-    return bf16_to_f32(p[i]) + sh;
+    return bf16_to_f32(elem(p, i)) + sh;
   }
   __device__ static void st(T* p, uint32_t i, C v, C sh) {
-    __builtin_nontemporal_store(f32_to_bf16_rne(v - sh), p + i);
+    __builtin_nontemporal_store(f32_to_bf16_rne(v - sh), elem_ptr(p, i));
   }
 };
 

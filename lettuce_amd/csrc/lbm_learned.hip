@@ -187,7 +187,8 @@ struct LearnedModel<D3Q27> {
   }
   // t <- t - M feq(rho, u) for the quadratic equilibrium
   // feq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u): each raw moment is
-  // rho (c0 P3(abc) + 3 sum_d u_d P3(abc + d) + 4.5 sum_dd' u_d u_d' P3(abc + d + d'))
+  // rho (c0 P3(abc) + 3 sum_d u_d P3(abc + d) + 4.5 sum_dd' u_d u_d' P3(abc + d + d'));
+  // orders 0 and 1 are left as they are (relax gives them weight 0)
   __device__ static void subtract_equilibrium(float (&t)[3][3][3], float rho,
                                               const float (&u)[3]) {
     const float uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -203,6 +204,7 @@ struct LearnedModel<D3Q27> {
       for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
+          if (a + b + c <= 1) continue;  // weight 0 in relax
           float m = 0.f;
           if (P3(a, b, c) != 0.f) m += P3(a, b, c) * c0;
           if (P3(a + 1, b, c) != 0.f) m += P3(a + 1, b, c) * lx;
@@ -227,45 +229,60 @@ struct LearnedModel<D3Q27> {
     x[4] = t[1][0][1] * sc;
     x[5] = t[0][1][1] * sc;
   }
-  // f <- f - Phi(w . t)
+  // f <- f - Phi(w . t).  The 1D backward transform is B' D with
+  // D = diag(1, 1/2, 1/2) and B' (m0, m1, m2) -> (m2 - m1, m0 - 2 m2, m1 + m2);
+  // the three D factors are powers of two and fold exactly into the weights
+  // as 2^-(number of non-zero exponents), which leaves 3 operations per 1D
+  // transform instead of 5.  Moments of order 0 and 1 have weight 0.
   __device__ static void relax(float (&f)[27], float (&t)[3][3][3], const float (&rate)[3],
                                float tau_inv) {
+    // weights of (order, non-zero exponents) (2,1) (2,2) (3,2) (3,3) (4,2) (4,3) (5|6,3)
+    const float w21 = 0.5f * tau_inv, w22 = 0.25f * tau_inv;
+    const float w32 = 0.25f * rate[0], w33 = 0.125f * rate[0];
+    const float w42 = 0.25f * rate[1], w43 = 0.125f * rate[1];
+    const float w53 = 0.125f * rate[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const int o = a + b + c;
-          t[a][b][c] *= o <= 1 ? 0.f
-                        : (o == 2 ? tau_inv : (o == 3 ? rate[0] : (o == 4 ? rate[1] : rate[2])));
+          const int o = a + b + c, nz = (a > 0) + (b > 0) + (c > 0);
+          if (o <= 1) {
+            t[a][b][c] = 0.f;
+            continue;
+          }
+          t[a][b][c] *= o == 2 ? (nz == 1 ? w21 : w22)
+                        : o == 3 ? (nz == 2 ? w32 : w33)
+                        : o == 4 ? (nz == 2 ? w42 : w43)
+                                 : w53;
         }
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float m0 = t[0][j][k], m1 = t[1][j][k], m2 = t[2][j][k];
-        t[0][j][k] = 0.5f * (m2 - m1);
-        t[1][j][k] = m0 - m2;
-        t[2][j][k] = 0.5f * (m1 + m2);
+        t[0][j][k] = m2 - m1;
+        t[1][j][k] = __builtin_fmaf(-2.f, m2, m0);
+        t[2][j][k] = m1 + m2;
       }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float m0 = t[i][0][k], m1 = t[i][1][k], m2 = t[i][2][k];
-        t[i][0][k] = 0.5f * (m2 - m1);
-        t[i][1][k] = m0 - m2;
-        t[i][2][k] = 0.5f * (m1 + m2);
+        t[i][0][k] = m2 - m1;
+        t[i][1][k] = __builtin_fmaf(-2.f, m2, m0);
+        t[i][2][k] = m1 + m2;
       }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const float m0 = t[i][j][0], m1 = t[i][j][1], m2 = t[i][j][2];
-        t[i][j][0] = 0.5f * (m2 - m1);
-        t[i][j][1] = m0 - m2;
-        t[i][j][2] = 0.5f * (m1 + m2);
+        t[i][j][0] = m2 - m1;
+        t[i][j][1] = __builtin_fmaf(-2.f, m2, m0);
+        t[i][j][2] = m1 + m2;
       }
 #pragma unroll
     for (int q = 0; q < 27; ++q) f[q] -= t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];

@@ -196,7 +196,7 @@ __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const
         idx = cell;
       }
     }
-    raw[q] = fin[pop * p.sq + idx];
+    raw[q] = (MASK && PULL) ? fin[pop * p.sq + idx] : elem(fin + q * p.sq, (uint32_t)idx);
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
