@@ -36,7 +36,7 @@ WORKLOADS = {
     "tgv3d_d3q27_learned_256_f32": ("D3Q27", "learned", "f32", 256, 1600, 0.1),
     "tgv3d_d3q27_learned_256_bf16s": ("D3Q27", "learned", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_none_256_f32": ("D3Q19", "none", "f32", 256, 1600, 0.1),
-    # the largest D3Q19 FP32 grid one device holds (two 76 GB arrays, `soa`)
+    # a D3Q19 FP32 grid near the largest one device holds (two 76 GB arrays)
     "tgv3d_d3q19_bgk_1024_f32": ("D3Q19", "bgk", "f32", 1024, 1600, 0.1),
     "tgv2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.1),
     # channel + cylinder (D = 64, halfway bounce-back, equilibrium in/outlet)
@@ -217,11 +217,7 @@ def main():
 
     import lettuce_amd as lt
     if args.layout is None:
-        # `rows` interleaves the Q arrays of a row, so its 32-bit cell
-        # offsets limit it to about 480^3 D3Q19 cells
-        res = WORKLOADS[args.workload][3]
-        big = isinstance(res, int) and res ** 3 * 27 >= 2 ** 31
-        args.layout = "soa" if args.workload.startswith("cyl2d") or big else "rows"
+        args.layout = "soa" if args.workload.startswith("cyl2d") else "rows"
     sim, S = build_sim(lt, args.workload, args.layout)
     stencil, coll, storage, res, re, ma = WORKLOADS[args.workload]
     cells = int(np.prod(sim.flow.resolution))

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -19,7 +20,8 @@ KernelFn pick_kernel_d2q9(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q19(int storage, int coll, bool mask, int mode);
 KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
 int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int storage, bool mask,
-                   int mode, hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out);
+                   int mode, bool wide, hipStream_t stream, int64_t nsteps, const void* f_in,
+                   void* f_out);
 KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q19(int storage, int coll, int mode, bool nt);
 KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
@@ -116,11 +118,34 @@ static int check_layout(int stencil, int storage, const int64_t n[3], const int6
     if (str[a] < span) return fail(LBM_E_ARG, "overlapping population strides");
     span = str[a] * ext[a];
   }
-  // cell offsets are 32-bit inside the kernels; populations use 64-bit bases
-  const int64_t cell_span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
-  if (cell_span > (int64_t)0x7fffffff) return fail(LBM_E_ARG, "grid too large for 32-bit cell offsets");
+  // Inside the kernels a cell's offset is its i0 plane's offset (64-bit,
+  // added to the population's base) plus a 32-bit offset inside the plane,
+  // which formats of up to 4 bytes address in bytes: the offsets inside a
+  // plane stay below 2^31 elements and below 2^32 bytes.
+  const int64_t plane_span = (n[1] - 1) * stride[2] + n[2];
+  const int bytes = storage_bytes(storage);
+  const int64_t plane_limit =
+      bytes <= 4 ? std::min<int64_t>((int64_t)1 << 31, ((int64_t)1 << 32) / bytes)
+                 : ((int64_t)1 << 31);
+  if (plane_span > plane_limit)
+    return fail(LBM_E_ARG, "an i0 plane spans %lld elements, above the %lld that 32-bit offsets inside a plane address",
+                (long long)plane_span, (long long)plane_limit);
   if (span > ((int64_t)1 << 40)) return fail(LBM_E_ARG, "population array too large");
   return LBM_OK;
+}
+
+// Whether a layout needs the wide (64-bit i0 plane) offsets of the
+// two-step and learned kernels: their narrow variants address a cell by one
+// 32-bit offset, in bytes for formats of up to 4 bytes.
+// LETTUCE_AMD_WIDE=1 selects the wide variants for every grid (tests run
+// them on small grids against the narrow ones).
+static bool wide_offsets(int storage, const int64_t n[3], const int64_t stride[4]) {
+  if (env_int("LETTUCE_AMD_WIDE", 0) == 1) return true;
+  const int64_t span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
+  const int bytes = storage_bytes(storage);
+  const int64_t limit = bytes <= 4 ? std::min<int64_t>((int64_t)1 << 31, ((int64_t)1 << 32) / bytes)
+                                   : ((int64_t)1 << 31);
+  return span > limit;
 }
 
 // masked kernels index the class map by the storage offset of a cell, which
@@ -175,7 +200,8 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
   return pick_tb2_kernel(a->stencil, a->storage, a->collision,
-                         env_int("LETTUCE_AMD_TB2_BARRIERS", 1) == 2 ? 2 : 1);
+                         env_int("LETTUCE_AMD_TB2_BARRIERS", 1) == 2 ? 2 : 1,
+                         wide_offsets(a->storage, a->n, a->stride));
 }
 
 // pairs of steps fused into one pass by the two-step kernel.  After nsteps
@@ -206,6 +232,10 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
     return fail(LBM_E_ARG, "cell_class given without a class table of 1..256 entries");
   if (mask && !contiguous_cells(a->n, a->stride))
     return fail(LBM_E_UNSUPPORTED, "boundaries need each population stored as one contiguous block");
+  // the class map and the boundary-value field are indexed by 32-bit cell
+  // offsets
+  if (mask && a->n[0] * a->n[1] * a->n[2] > (int64_t)0x7fffffff)
+    return fail(LBM_E_ARG, "grids with boundaries are limited to 2^31 - 1 cells");
   if (a->collision == LBM_COLL_BGK && !(a->tau > 0)) return fail(LBM_E_ARG, "BGK needs tau > 0");
   if (a->collision == LBM_COLL_BGK_GUO) {
     if (!(a->tau > 0) || !(a->force_tau > 0))
@@ -253,7 +283,7 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.n0 = (int)a->n[0];
   p.n1 = (int)a->n[1];
   p.n2 = (int)a->n[2];
-  p.s0 = (int)a->stride[1];
+  p.s0 = a->stride[1];
   p.s1 = (int)a->stride[2];
   p.sq = a->stride[0];
   p.ncells = a->n[0] * a->n[1] * a->n[2];
@@ -353,7 +383,8 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   g_err[0] = 0;
   if (a->collision == LBM_COLL_LEARNED) {
     rc = launch_learned(p, a->learned, a->stencil, a->storage, a->cell_class != nullptr, a->streaming,
-                        (hipStream_t)stream, nsteps, a->f_in, a->f_out);
+                        wide_offsets(a->storage, a->n, a->stride), (hipStream_t)stream, nsteps,
+                        a->f_in, a->f_out);
     if (rc) return fail(rc, "no learned kernel for storage %d", a->storage);
     return check_launch("lbm_run (learned)");
   }
@@ -425,7 +456,7 @@ int lbm_halo_pack(int32_t stencil, int32_t storage, const int64_t n[3], const in
   p.n0 = (int)n[0];
   p.n1 = (int)n[1];
   p.n2 = (int)n[2];
-  p.s0 = (int)stride[1];
+  p.s0 = stride[1];
   p.s1 = (int)stride[2];
   p.sq = stride[0];
   const int bs = block_size_for(n[2]);
@@ -509,7 +540,7 @@ __device__ __forceinline__ void load_cell(const Grid& g, const void* f, int64_t 
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
     const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
-    v[q] = ST::ld((const T*)f + q * g.sq, (uint32_t)off, sh);
+    v[q] = ST::ld((const T*)f + q * g.sq + off, 0, sh);
   }
 }
 
@@ -527,7 +558,7 @@ __global__ void k_equilibrium(Grid g, const IN* rho, const IN* u, double rho0, v
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
       const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
-      ST::st((T*)f + q * g.sq, (uint32_t)off, fe[q], sh);
+      ST::st((T*)f + q * g.sq + off, 0, fe[q], sh);
     }
   }
 }
@@ -554,7 +585,7 @@ __global__ void k_encode(Grid g, const IN* in, double rho0, void* f) {
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
       const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
-      ST::st((T*)f + q * g.sq, (uint32_t)off, (C)in[q * g.cells + c], sh);
+      ST::st((T*)f + q * g.sq + off, 0, (C)in[q * g.cells + c], sh);
     }
   }
 }
@@ -764,7 +795,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_link_force(Grid g, const void* f,
     for (int q = 0; q < S::Q; ++q)
       if (q == d) {
         const C sh = ST::shifted ? (C)(S::w[q] * rho0) : C(0);
-        v = (double)ST::ld((const T*)f + q * g.sq, (uint32_t)off, sh);
+        v = (double)ST::ld((const T*)f + q * g.sq + off, 0, sh);
         acc[0] += 2.0 * v * S::e[q][0];
         acc[1] += 2.0 * v * S::e[q][1];
         acc[2] += 2.0 * v * S::e[q][2];

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/lettuce_amd.h"
 
 namespace lbm {
@@ -85,25 +87,35 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// Element i of a population array p (a pointer uniform across the wave).
-// Element offsets are below 2^31 (checked on the host), so for 16-bit
-// formats the byte offset 2 i fits 32 bits: p stays in scalar registers
-// and each access takes one 32-bit vector offset (the global_load saddr
-// form) instead of a 64-bit address computed per lane.  Wider formats keep
-// 64-bit addresses (4 i may exceed 2^32).
+// Element i of a population array p, where p (the population's base plus
+// the offset of an i0 plane) is uniform across the wave and i is an offset
+// inside the plane.  For formats of up to 4 bytes the host keeps the byte
+// offsets inside a plane below 2^32, so p stays in scalar registers and
+// each access takes one 32-bit vector offset (the global_load saddr form)
+// instead of a 64-bit address computed per lane.  FP64 keeps 64-bit
+// addresses.  The accesses go through global (address space 1) pointers.
 template <class T>
-__device__ __forceinline__ const T& elem(const T* p, uint32_t i) {
-  if constexpr (sizeof(T) == 2)
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i << 1));
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> elem_ptr(T* p, uint32_t i) {
+  if constexpr (sizeof(T) <= 4)
+    return (gptr<T>)((gptr<char>)p + i * (uint32_t)sizeof(T));
   else
-    return p[i];
+    return (gptr<T>)p + i;
 }
 template <class T>
-__device__ __forceinline__ T* elem_ptr(T* p, uint32_t i) {
-  if constexpr (sizeof(T) == 2)
-    return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (i << 1));
-  else
-    return p + i;
+__device__ __forceinline__ T elem(const T* p, uint32_t i) {
+  return *elem_ptr(const_cast<T*>(p), i);
+}
+// p kept whole in scalar registers: in a loop whose in-plane offsets do not
+// change (the two-step kernel's sweep over planes), the compiler would
+// otherwise hoist base + offset out of the loop as one 64-bit vector
+// register pair per population and add the plane offset per access
+template <class T>
+__device__ __forceinline__ T* scalar_base(T* p) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v));
+  return (T*)v;
 }
 
 // dec: stored value -> population, enc: population -> stored value;
@@ -112,17 +124,19 @@ struct StF32 {
   using T = float;
   using C = float;
   static constexpr bool shifted = false;
+  __device__ static C val(T x) { return x; }  // without the shift
   __device__ static C dec(T x, C) { return x; }
   __device__ static T enc(C v, C) { return v; }
-  __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
+  __device__ static C ld(const T* p, uint32_t i, C) { return elem(p, i); }
   __device__ static void st(T* p, uint32_t i, C v, C) {
-    __builtin_nontemporal_store(v, p + i);
+    __builtin_nontemporal_store(v, elem_ptr(p, i));
   }
 };
 struct StF64 {
   using T = double;
   using C = double;
   static constexpr bool shifted = false;
+  __device__ static C val(T x) { return x; }  // without the shift
   __device__ static C dec(T x, C) { return x; }
   __device__ static T enc(C v, C) { return v; }
   __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
@@ -134,6 +148,7 @@ struct StF16 {
   using T = _Float16;
   using C = float;
   static constexpr bool shifted = true;
+  __device__ static C val(T x) { return (float)x; }  // without the shift
   __device__ static C dec(T x, C sh) { return (float)x + sh; }
   __device__ static T enc(C v, C sh) { return (_Float16)(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
@@ -147,6 +162,7 @@ struct StBF16 {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
+  __device__ static C val(T x) { return bf16_to_f32(x); }  // without the shift
   __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
   __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
   __device__ static C ld(const T* p, uint32_t i, C sh) {

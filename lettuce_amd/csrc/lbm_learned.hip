@@ -185,6 +185,16 @@ struct LearnedModel<D3Q27> {
   __host__ __device__ static constexpr float P3(int a, int b, int c) {
     return mu1(a) * mu1(b) * mu1(c);
   }
+  // t <- t + M (w rho0): the raw moments of the shift that shifted formats
+  // leave out of the stored values, rho0 P3(abc) where a, b, c are all even
+  __device__ static void add_shift(float (&t)[3][3][3], double rho0) {
+#pragma unroll
+    for (int a = 0; a < 3; a += 2)
+#pragma unroll
+      for (int b = 0; b < 3; b += 2)
+#pragma unroll
+        for (int c = 0; c < 3; c += 2) t[a][b][c] += (float)(rho0 * P3(a, b, c));
+  }
   // t <- t - M feq(rho, u) for the quadratic equilibrium
   // feq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u): each raw moment is
   // rho (c0 P3(abc) + 3 sum_d u_d P3(abc + d) + 4.5 sum_dd' u_d u_d' P3(abc + d + d'));
@@ -289,8 +299,10 @@ struct LearnedModel<D3Q27> {
   }
 };
 
-template <class S, class ST, bool MASK, int MODE>
-__global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const LearnedArgs la) {
+// WIDE: 64-bit i0 plane offsets, for arrays whose offsets (in bytes, for
+// formats of up to 4 bytes) do not fit 32 bits
+template <class S, class ST, bool MASK, int MODE, bool WIDE>
+__global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const LearnedArgs la) {
   using C = float;
   constexpr int Q = S::Q;
   constexpr bool PULL = (MODE & LBM_PRE_STREAMING) != 0;
@@ -329,29 +341,44 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
   };
 
   // the tile loop is uniform across the wave: every MFMA runs with all lanes
-  for (int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < la.ntiles;
-       tile += nwaves) {
+  // (readfirstlane tells the compiler so: the tile's i0 plane offset then
+  // lives in scalar registers)
+  for (int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                 (threadIdx.x >> 6));
+       tile < la.ntiles; tile += nwaves) {
     const int b2 = tile % la.nt2;
     const int r = tile / la.nt2;
     const int i1 = r % p.n1, i0 = r / p.n1;
     const int i2 = b2 * 64 + lane;
-    int a0[3], a1[3], a2[3];
-    axis_offsets(i0, p.n0, p.s0, a0);
+    using X = std::conditional_t<WIDE, long long, int>;
+    X a0[3];
+    int a1[3], a2[3];
+    axis_offsets(i0, p.n0, (X)p.s0, a0);
     axis_offsets(i1, p.n1, p.s1, a1);
     axis_offsets(i2, p.n2, 1, a2);
-    const int cell = a0[1] + a1[1] + a2[1];
+    const int cell = (int)(a0[1] + a1[1] + a2[1]);  // used with boundaries only
+    using LM = LearnedModel<S>;
+    // D3Q27 with a shifted format works on the stored values g = f - w rho0
+    // throughout: the update f' = f - Phi(...) is g' = g - Phi(...), and the
+    // shift enters only the moments (add_shift).  The KBC fallback and the
+    // boundary ops decode and re-encode.
+    constexpr bool RAW = LM::kMomentSpace && ST::shifted;
+    auto shift_all = [&](C (&v)[Q], C sign) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[q] += sign * shift_of<ST>(S::w[q], p);
+    };
     uint32_t nsm, flags, op;
     C f[Q];
-    gather_batched<S, ST, MASK, PULL>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
+    gather_batched<S, ST, MASK, PULL, RAW>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
 
     // non-equilibrium part and the MLP inputs
-    using LM = LearnedModel<S>;
     C feat[6];
     // D3Q27: the raw moments of df (mt); D2Q9: the KBC split (ds, dh)
     C mt[LM::kMomentSpace ? 3 : 1][3][3], ds[LM::kMomentSpace ? 1 : Q], dh[LM::kMomentSpace ? 1 : Q];
     if constexpr (LM::kMomentSpace) {
       // rho and j are moments 000 and 100 / 010 / 001 of f
       LM::to_moments(f, mt);
+      if constexpr (RAW) LM::add_shift(mt, p.rho0);
       const C rho = mt[0][0][0], ri = recip(rho);
       const C u[3] = {mt[1][0][0] * ri, mt[0][1][0] * ri, mt[0][0][1] * ri};
       LM::subtract_equilibrium(mt, rho, u);
@@ -413,56 +440,82 @@ __global__ void __launch_bounds__(256, 2) learned_kernel(const KParams p, const 
         rate[g] = fminf(fmaxf(raw, la.rate_lo), la.rate_hi);
       }
       if (fallback) {
+        if constexpr (RAW) shift_all(f, C(1));
         collide_kbc<S, C>(f, cp);
+        if constexpr (RAW) shift_all(f, C(-1));
       } else if constexpr (LM::kMomentSpace) {
         LM::relax(f, mt, rate, tau_inv);
       } else {
         LM::update(f, ds, dh, rate, tau_inv);
       }
     } else {
+      if constexpr (RAW) shift_all(f, C(1));
       apply_boundary_op<S, ST, PULL>(p, op, flags, cell, i0, i1, i2, f);
+      if constexpr (RAW) shift_all(f, C(-1));
     }
-    if (!out_of_grid) store_cell<S, ST, MASK, PUSH>(p, cell, a0, a1, a2, nsm, flags, f);
+    if (out_of_grid) continue;
+    if constexpr (!PUSH) {
+      // the cell's own slots through one per-lane pointer stepped by the
+      // population stride: the gather's per-population bases already fill
+      // the scalar registers, and 27 more would spill
+      using T = typename ST::T;
+      gptr<T> o = (gptr<T>)p.fout + (a0[1] + (long long)(a1[1] + a2[1]));
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        __builtin_nontemporal_store(ST::enc(f[q], RAW ? C(0) : shift_of<ST>(S::w[q], p)), o);
+        o += p.sq;
+        asm volatile("" : "+v"(o));
+      }
+    } else {
+      store_cell<S, ST, MASK, PUSH, RAW>(p, cell, a0, a1, a2, nsm, flags, f);
+    }
   }
 }
 
 using LearnedFn = void (*)(const KParams, const LearnedArgs);
 
-template <class S, class ST, bool MASK>
+template <class S, class ST, bool MASK, bool WIDE>
 LearnedFn learned_mode(int mode) {
   switch (mode) {
-    case 0: return learned_kernel<S, ST, MASK, 0>;
-    case 1: return learned_kernel<S, ST, MASK, 1>;
-    case 2: return learned_kernel<S, ST, MASK, 2>;
-    case 3: return learned_kernel<S, ST, MASK, 3>;
+    case 0: return learned_kernel<S, ST, MASK, 0, WIDE>;
+    case 1: return learned_kernel<S, ST, MASK, 1, WIDE>;
+    case 2: return learned_kernel<S, ST, MASK, 2, WIDE>;
+    case 3: return learned_kernel<S, ST, MASK, 3, WIDE>;
   }
   return nullptr;
 }
 
-template <class S, class ST>
+template <class S, class ST, bool WIDE>
 LearnedFn learned_mask(bool mask, int mode) {
-  return mask ? learned_mode<S, ST, true>(mode) : learned_mode<S, ST, false>(mode);
+  return mask ? learned_mode<S, ST, true, WIDE>(mode) : learned_mode<S, ST, false, WIDE>(mode);
 }
 
-template <class S>
+template <class S, bool WIDE>
 LearnedFn learned_storage(int storage, bool mask, int mode) {
   switch (storage) {
-    case LBM_F32: return learned_mask<S, StF32>(mask, mode);
-    case LBM_F16_SHIFTED: return learned_mask<S, StF16>(mask, mode);
-    case LBM_BF16_SHIFTED: return learned_mask<S, StBF16>(mask, mode);
+    case LBM_F32: return learned_mask<S, StF32, WIDE>(mask, mode);
+    case LBM_F16_SHIFTED: return learned_mask<S, StF16, WIDE>(mask, mode);
+    case LBM_BF16_SHIFTED: return learned_mask<S, StBF16, WIDE>(mask, mode);
   }
   return nullptr;  // FP64 storage: the learned collision computes in FP32
 }
 
-LearnedFn pick_learned_kernel(int stencil, int storage, bool mask, int mode) {
-  if (stencil == LBM_D2Q9) return learned_storage<D2Q9>(storage, mask, mode);
-  if (stencil == LBM_D3Q27) return learned_storage<D3Q27>(storage, mask, mode);
+template <class S>
+LearnedFn learned_wide(int storage, bool mask, int mode, bool wide) {
+  return wide ? learned_storage<S, true>(storage, mask, mode)
+              : learned_storage<S, false>(storage, mask, mode);
+}
+
+LearnedFn pick_learned_kernel(int stencil, int storage, bool mask, int mode, bool wide) {
+  if (stencil == LBM_D2Q9) return learned_wide<D2Q9>(storage, mask, mode, wide);
+  if (stencil == LBM_D3Q27) return learned_wide<D3Q27>(storage, mask, mode, wide);
   return nullptr;
 }
 
 int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int storage, bool mask,
-                   int mode, hipStream_t stream, int64_t nsteps, const void* f_in, void* f_out) {
-  LearnedFn fn = pick_learned_kernel(stencil, storage, mask, mode);
+                   int mode, bool wide, hipStream_t stream, int64_t nsteps, const void* f_in,
+                   void* f_out) {
+  LearnedFn fn = pick_learned_kernel(stencil, storage, mask, mode, wide);
   if (!fn) return LBM_E_UNSUPPORTED;
   LearnedArgs la;
   la.packed = (const LearnedPacked*)l->packed;

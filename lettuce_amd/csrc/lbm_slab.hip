@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) face_kernel(const KParams p, const SlabHa
   axis_offsets(i1, p.n1, p.s1, a1);  // rows of f
   axis_offsets(i1, p.n1, p.n2, hr);  // rows of a halo plane
   axis_offsets(i2, p.n2, 1, a2);
-  const int cell = i0 * p.s0 + a1[1] + a2[1];
+  const long long own_plane = i0 * p.s0;  // offset of plane i0 in f
   const uint32_t plane = (uint32_t)p.n1 * (uint32_t)p.n2;
   const T* fin = (const T*)p.fin;
   const T* rlo = (const T*)h.recv_lo;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) face_kernel(const KParams p, const SlabHa
     else if (ex == -1 && xs >= p.n0)
       f[q] = ST::ld(rhi + cross_index<S>(q) * plane, (uint32_t)(hr[ry] + a2[rz]), sh);
     else
-      f[q] = ST::ld(fin + q * p.sq, (uint32_t)(xs * p.s0 + a1[ry] + a2[rz]), sh);
+      f[q] = ST::ld(fin + q * p.sq + xs * p.s0, (uint32_t)(a1[ry] + a2[rz]), sh);
   }
 
   const CollParams<C> cp = coll_params<C>(p);
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) face_kernel(const KParams p, const SlabHa
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
     const C sh = shift_of<ST>(S::w[q], p);
-    ST::st(fout + q * p.sq, (uint32_t)cell, f[q], sh);
+    ST::st(fout + q * p.sq + own_plane, (uint32_t)(a1[1] + a2[1]), f[q], sh);
     // the same encoded value leaves through the face it points at
     if (go<S>(q, 0) == -1 && i0 == 0) ST::st(slo + cross_index<S>(q) * plane, own, f[q], sh);
     if (go<S>(q, 0) == 1 && i0 == p.n0 - 1) ST::st(shi + cross_index<S>(q) * plane, own, f[q], sh);
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) pack_kernel(const KParams p, const SlabHa
   face_tile(p, blockIdx.x, i0, i1, b2);
   const int i2 = b2 * blockDim.x + threadIdx.x;
   if (i2 >= p.n2) return;
-  const int cell = i0 * p.s0 + i1 * p.s1 + i2;
+  const long long cell = i0 * p.s0 + i1 * p.s1 + i2;
   const uint32_t plane = (uint32_t)p.n1 * (uint32_t)p.n2;
   const uint32_t own = (uint32_t)(i1 * p.n2 + i2);
   const U* fin = (const U*)p.fin;

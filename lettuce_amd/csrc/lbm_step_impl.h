@@ -25,7 +25,8 @@ struct KParams {
   const lbm_cell_class* ct;
   const double* bcf;
   int n0, n1, n2;     // extents
-  int s0, s1;         // cell strides of i0 and i1 (elements); i2 stride is 1
+  long long s0;       // cell stride of i0 (elements)
+  int s1;             // cell stride of i1 (elements); i2 stride is 1
   long long sq;       // population stride (elements)
   long long ncells;   // n0*n1*n2 (bc_field plane stride)
   int nb2;            // blocks along i2
@@ -81,19 +82,53 @@ __device__ __forceinline__ CollParams<C> coll_params(const KParams& p) {
 }
 
 // periodic neighbour offsets along one axis: a[0] = i-1, a[1] = i, a[2] = i+1
-// (wrapped), each multiplied by the axis stride
-__device__ __forceinline__ void axis_offsets(int i, int n, int s, int (&a)[3]) {
-  a[0] = (i == 0 ? n - 1 : i - 1) * s;
-  a[1] = i * s;
-  a[2] = (i == n - 1 ? 0 : i + 1) * s;
+// (wrapped), each multiplied by the axis stride.  Offsets along i0 are
+// 64-bit: a cell's offset is its i0 plane's offset (uniform across a block,
+// added to the population's base pointer in scalar registers) plus a 32-bit
+// offset inside the plane, so only a plane must stay below 2^31 elements.
+template <class I>
+__device__ __forceinline__ void axis_offsets(int i, int n, I s, I (&a)[3]) {
+  a[0] = (I)(i == 0 ? n - 1 : i - 1) * s;
+  a[1] = (I)i * s;
+  a[2] = (I)(i == n - 1 ? 0 : i + 1) * s;
 }
 
-// offset of x + sign * e_q for a cell with axis offset tables a0, a1, a2
+// offset of x + sign * e_q inside its i0 plane, and of its i0 plane
 template <class S>
-__device__ __forceinline__ int nb_offset(int q, int sign, const int (&a0)[3],
+__device__ __forceinline__ int nb_in_plane(int q, int sign, const int (&a1)[3],
+                                           const int (&a2)[3]) {
+  return a1[1 + sign * go<S>(q, 1)] + a2[1 + sign * go<S>(q, 2)];
+}
+template <class S, class X>
+__device__ __forceinline__ X nb_plane(int q, int sign, const X (&a0)[3]) {
+  return a0[1 + sign * go<S>(q, 0)];
+}
+// A kernel keeps its i0 offsets (X) 64-bit where the array needs it: the
+// plane offset then goes into the population's base pointer.  With 32-bit
+// i0 offsets (arrays whose offsets fit 32 bits) it goes into the 32-bit
+// offset, which leaves the per-population bases unchanged across cells.
+template <class T, class X>
+__device__ __forceinline__ T* at_plane(T* base, X plane) {
+  if constexpr (std::is_same_v<X, int>) return base;
+  else return base + plane;
+}
+template <class X>
+__device__ __forceinline__ uint32_t in_plane(X plane, int in) {
+  if constexpr (std::is_same_v<X, int>) return (uint32_t)(plane + in);
+  else return (uint32_t)in;
+}
+// full offset of x + sign * e_q; the masked kernels' cell offsets (class map
+// lookups) are below 2^31, which the host checks for grids with boundaries
+template <class S>
+__device__ __forceinline__ int nb_offset(int q, int sign, const long long (&a0)[3],
                                          const int (&a1)[3], const int (&a2)[3]) {
-  return a0[1 + sign * go<S>(q, 0)] + a1[1 + sign * go<S>(q, 1)] +
-         a2[1 + sign * go<S>(q, 2)];
+  return (int)(nb_plane<S>(q, sign, a0) + nb_in_plane<S>(q, sign, a1, a2));
+}
+// the same with 32-bit i0 offsets (grids of fewer than 2^31 populations)
+template <class S>
+__device__ __forceinline__ int nb_offset(int q, int sign, const int (&a0)[3], const int (&a1)[3],
+                                         const int (&a2)[3]) {
+  return a0[1 + sign * go<S>(q, 0)] + nb_in_plane<S>(q, sign, a1, a2);
 }
 
 template <class ST>
@@ -110,7 +145,7 @@ __device__ __forceinline__ bool cls_solid(const KParams& p, int cell) {
 // tables are a0/a1/a2, with the cell's own class fields (nsm, flags).
 template <class S, class ST, bool MASK>
 __device__ __forceinline__ void gather_pull(const KParams& p, int cell,
-                                            const int (&a0)[3], const int (&a1)[3],
+                                            const long long (&a0)[3], const int (&a1)[3],
                                             const int (&a2)[3], uint32_t nsm,
                                             uint32_t flags, typename ST::C (&f)[S::Q]) {
   using T = typename ST::T;
@@ -118,10 +153,12 @@ __device__ __forceinline__ void gather_pull(const KParams& p, int cell,
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
     const typename ST::C sh = shift_of<ST>(S::w[q], p);
-    const int src = nb_offset<S>(q, -1, a0, a1, a2);
     if (!MASK) {
-      f[q] = ST::ld(fin + q * p.sq, src, sh);
+      const long long plane = nb_plane<S>(q, -1, a0);
+      f[q] = ST::ld(at_plane(fin + q * p.sq, plane), in_plane(plane, nb_in_plane<S>(q, -1, a1, a2)),
+                    sh);
     } else {
+      const int src = nb_offset<S>(q, -1, a0, a1, a2);
       bool hw = false;
       if ((flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID && q != 0)
         hw = cls_solid(p, src);
@@ -153,17 +190,24 @@ __device__ __forceinline__ void stage_classes(const KParams& p, lbm_cell_class* 
 // scheduler interleaves loads with their uses to save registers, which
 // serialises the memory round trips of these latency-bound kernels.
 // Returns the cell's no-stream bits, flags and op in nsm, flags, op.
-template <class S, class ST, bool MASK, bool PULL>
-__device__ __forceinline__ void gather_batched(const KParams& p, int cell, const int (&a0)[3],
+// RAW: f receives the stored values without the shift of shifted formats.
+template <class S, class ST, bool MASK, bool PULL, bool RAW = false, class X>
+__device__ __forceinline__ void gather_batched(const KParams& p, int cell, const X (&a0)[3],
                                                const int (&a1)[3], const int (&a2)[3],
                                                const lbm_cell_class* ctl, uint32_t& nsm,
                                                uint32_t& flags, uint32_t& op,
                                                typename ST::C (&f)[S::Q]) {
   using T = typename ST::T;
   const T* fin = (const T*)p.fin;
-  int src[S::Q];
+  // pull sources: plane and in-plane offsets (masked: the full offset)
+  X splane[S::Q];
+  int sinp[S::Q], src[S::Q];
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) src[q] = PULL ? nb_offset<S>(q, -1, a0, a1, a2) : cell;
+  for (int q = 0; q < S::Q; ++q) {
+    splane[q] = PULL ? nb_plane<S>(q, -1, a0) : a0[1];
+    sinp[q] = PULL ? nb_in_plane<S>(q, -1, a1, a2) : a1[1] + a2[1];
+    src[q] = MASK ? (int)(splane[q] + sinp[q]) : 0;
+  }
   uint32_t c0 = 0, cid[S::Q];
   if (MASK) {
     c0 = p.cls[cell];
@@ -196,20 +240,23 @@ __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const
         idx = cell;
       }
     }
-    raw[q] = (MASK && PULL) ? fin[pop * p.sq + idx] : elem(fin + q * p.sq, (uint32_t)idx);
+    raw[q] = (MASK && PULL) ? fin[pop * p.sq + idx]
+                            : elem(at_plane(fin + q * p.sq, splane[q]), in_plane(splane[q], sinp[q]));
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+  for (int q = 0; q < S::Q; ++q)
+    f[q] = RAW ? ST::val(raw[q]) : ST::dec(raw[q], shift_of<ST>(S::w[q], p));
 }
 
+// the populations of the cell at offset plane + in (its i0 plane, and inside it)
 template <class S, class ST>
-__device__ __forceinline__ void gather_own(const KParams& p, int cell,
+__device__ __forceinline__ void gather_own(const KParams& p, long long plane, int in,
                                            typename ST::C (&f)[S::Q]) {
   using T = typename ST::T;
-  const T* fin = (const T*)p.fin;
+  const T* fin = (const T*)p.fin + plane;
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) f[q] = ST::ld(fin + q * p.sq, cell, shift_of<ST>(S::w[q], p));
+  for (int q = 0; q < S::Q; ++q) f[q] = ST::ld(fin + q * p.sq, in, shift_of<ST>(S::w[q], p));
 }
 
 template <class S, class C, int COLL>
@@ -232,16 +279,17 @@ __device__ __forceinline__ void outlet_neighbour(const KParams& p, const lbm_cel
   const int w0 = j0 < 0 ? j0 + p.n0 : (j0 >= p.n0 ? j0 - p.n0 : j0);
   const int w1 = j1 < 0 ? j1 + p.n1 : (j1 >= p.n1 ? j1 - p.n1 : j1);
   const int w2 = j2 < 0 ? j2 + p.n2 : (j2 >= p.n2 ? j2 - p.n2 : j2);
-  int c0[3], c1[3], c2[3];
+  long long c0[3];
+  int c1[3], c2[3];
   axis_offsets(w0, p.n0, p.s0, c0);
   axis_offsets(w1, p.n1, p.s1, c1);
   axis_offsets(w2, p.n2, 1, c2);
-  const int nb = c0[1] + c1[1] + c2[1];
+  const int nb = (int)(c0[1] + c1[1] + c2[1]);
   if (PULL) {
     const lbm_cell_class& KN = p.ct[p.cls[nb]];
     gather_pull<S, ST, true>(p, nb, c0, c1, c2, KN.no_stream, KN.flags, g);
   } else {
-    gather_own<S, ST>(p, nb, g);
+    gather_own<S, ST>(p, 0, nb, g);
   }
 }
 
@@ -313,8 +361,9 @@ __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op,
 
 // Write the cell's post-collision populations: in place (no post-streaming)
 // or pushed to the neighbours with the streaming rules of the destination.
-template <class S, class ST, bool MASK, bool PUSH>
-__device__ __forceinline__ void store_cell(const KParams& p, int cell, const int (&a0)[3],
+// RAW: f holds values to store as they are (no shift subtracted).
+template <class S, class ST, bool MASK, bool PUSH, bool RAW = false, class X>
+__device__ __forceinline__ void store_cell(const KParams& p, int cell, const X (&a0)[3],
                                            const int (&a1)[3], const int (&a2)[3], uint32_t nsm,
                                            uint32_t flags, const typename ST::C (&f)[S::Q]) {
   using C = typename ST::C;
@@ -322,11 +371,15 @@ __device__ __forceinline__ void store_cell(const KParams& p, int cell, const int
   T* fout = (T*)p.fout;
   if (!PUSH) {
 #pragma unroll
-    for (int q = 0; q < S::Q; ++q) ST::st(fout + q * p.sq, cell, f[q], shift_of<ST>(S::w[q], p));
+    for (int q = 0; q < S::Q; ++q)
+      ST::st(at_plane(fout + q * p.sq, a0[1]), in_plane(a0[1], a1[1] + a2[1]), f[q],
+             RAW ? C(0) : shift_of<ST>(S::w[q], p));
   } else if (!MASK) {
 #pragma unroll
     for (int q = 0; q < S::Q; ++q)
-      ST::st(fout + q * p.sq, nb_offset<S>(q, +1, a0, a1, a2), f[q], shift_of<ST>(S::w[q], p));
+      ST::st(at_plane(fout + q * p.sq, nb_plane<S>(q, +1, a0)),
+             in_plane(nb_plane<S>(q, +1, a0), nb_in_plane<S>(q, +1, a1, a2)), f[q],
+             RAW ? C(0) : shift_of<ST>(S::w[q], p));
   } else {
     // Push with the pull rule evaluated at the destination d = x + e_q:
     //   d fluid and x solid (halfway bounce-back)  -> d fills itself
@@ -335,7 +388,7 @@ __device__ __forceinline__ void store_cell(const KParams& p, int cell, const int
     const bool self_solid = (flags & LBM_CLASS_SOLID) != 0;
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
-      const C sh = shift_of<ST>(S::w[q], p);
+      const C sh = RAW ? C(0) : shift_of<ST>(S::w[q], p);
       // (a) to the neighbour
       const int dst = nb_offset<S>(q, +1, a0, a1, a2);
       const lbm_cell_class& KD = p.ct[p.cls[dst]];
@@ -372,11 +425,12 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
   const int i2 = b2 * blockDim.x + threadIdx.x;
   if (i2 >= p.n2) return;
 
-  int a0[3], a1[3], a2[3];
+  long long a0[3];
+  int a1[3], a2[3];
   axis_offsets(i0, p.n0, p.s0, a0);
   axis_offsets(i1, p.n1, p.s1, a1);
   axis_offsets(i2, p.n2, 1, a2);
-  const int cell = a0[1] + a1[1] + a2[1];
+  const int cell = (int)(a0[1] + a1[1] + a2[1]);  // used with boundaries only
 
   uint32_t nsm = 0, flags = 0, op = LBM_OP_COLLIDE;
   C f[S::Q];
@@ -385,7 +439,7 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
   } else if (PULL) {
     gather_pull<S, ST, MASK>(p, cell, a0, a1, a2, nsm, flags, f);
   } else {
-    gather_own<S, ST>(p, cell, f);
+    gather_own<S, ST>(p, a0[1], a1[1] + a2[1], f);
   }
 
   const CollParams<C> cp = coll_params<C>(p);
@@ -410,7 +464,8 @@ __global__ void __launch_bounds__(256) pair_kernel(const KParams p) {
   block_tile(p, blockIdx.x, i0, i1, b2);
   const int z0 = 2 * (b2 * (int)blockDim.x + (int)threadIdx.x);
   if (z0 >= p.n2) return;
-  int a0[3], a1[3];
+  long long a0[3];
+  int a1[3];
   axis_offsets(i0, p.n0, p.s0, a0);
   axis_offsets(i1, p.n1, p.s1, a1);
   const bool lo_edge = z0 == 0, hi_edge = z0 + 2 == p.n2;
@@ -502,14 +557,15 @@ __global__ void __launch_bounds__(1024) row_kernel(const KParams p) {
   const int i1 = row % p.n1, i0 = row / p.n1;
   const int z = threadIdx.x, lane = z & 63, wid = z >> 6;
   const int nw = blockDim.x >> 6;
-  int a0[3], a1[3];
+  long long a0[3];
+  int a1[3];
   axis_offsets(i0, p.n0, p.s0, a0);
   axis_offsets(i1, p.n1, p.s1, a1);
   const T* fin = (const T*)p.fin;
   T raw[S::Q];
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
-    const T v = fin[q * p.sq + a0[1 - go<S>(q, 0)] + a1[1 - go<S>(q, 1)] + z];
+    const T v = (fin + q * p.sq + a0[1 - go<S>(q, 0)])[a1[1 - go<S>(q, 1)] + z];
     const int ez = go<S>(q, 2);
     if (ez == 0) {
       raw[q] = v;
@@ -533,9 +589,9 @@ __global__ void __launch_bounds__(1024) row_kernel(const KParams p) {
   for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
   collide<S, C, COLL>(f, coll_params<C>(p));
   T* fout = (T*)p.fout;
-  const int cell = a0[1] + a1[1] + z;
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) ST::st(fout + q * p.sq, cell, f[q], shift_of<ST>(S::w[q], p));
+  for (int q = 0; q < S::Q; ++q)
+    ST::st(fout + q * p.sq + a0[1], a1[1] + z, f[q], shift_of<ST>(S::w[q], p));
 }
 
 // ---------------------------------------------------------------------------

@@ -20,7 +20,8 @@ struct TbParams {
 using TbFn = void (*)(const TbParams);
 
 // two-step kernel for (stencil, storage, collision), or nullptr, with one
-// or two block-wide barriers per plane
-TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers);
+// or two block-wide barriers per plane; wide: for arrays whose offsets do
+// not fit 32 bits (one barrier per plane)
+TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers, bool wide);
 
 }  // namespace lbm

@@ -54,7 +54,11 @@ __host__ __device__ constexpr int tb_base(int q, int which) {
   return b;
 }
 
-template <class S, class ST, int COLL, int NBAR>
+// WIDE: cell offsets as a 64-bit plane offset plus a 32-bit offset inside
+// the plane, for arrays whose offsets (in bytes, for formats of up to 4
+// bytes) do not fit 32 bits; otherwise one 32-bit offset per access, which
+// keeps the per-population bases loop-invariant in scalar registers.
+template <class S, class ST, int COLL, int NBAR, bool WIDE>
 __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   using C = typename ST::C;
   using T = typename ST::T;
@@ -106,12 +110,17 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
     if (active) {
       int xa = x0 - 1 + it;
       xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
-      int ax[3];
-      axis_offsets(xa, p.n0, p.s0, ax);
+      using X = std::conditional_t<WIDE, long long, int>;
+      X ax[3];
+      axis_offsets(xa, p.n0, (X)p.s0, ax);
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) {
         const C sh = shift_of<ST>(S::w[q], p);
-        cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)], sh);
+        const int in = ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)];
+        if constexpr (WIDE)
+          cur[q] = ST::ld(scalar_base(fin + q * p.sq + ax[1 - go<S>(q, 0)]), in, sh);
+        else
+          cur[q] = ST::ld(fin + q * p.sq, ax[1 - go<S>(q, 0)] + in, sh);
       }
       collide<S, C, COLL>(cur, cp);
 #pragma unroll
@@ -161,9 +170,15 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         }
       }
       collide<S, C, COLL>(g, cp);
-      const int cell = xb * p.s0 + ay[1] + az[1];
+      const long long plane = xb * p.s0;
 #pragma unroll
-      for (int q = 0; q < S::Q; ++q) ST::st(fout + q * p.sq, cell, g[q], shift_of<ST>(S::w[q], p));
+      for (int q = 0; q < S::Q; ++q) {
+        if constexpr (WIDE)
+          ST::st(scalar_base(fout + q * p.sq + plane), ay[1] + az[1], g[q],
+                 shift_of<ST>(S::w[q], p));
+        else
+          ST::st(fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
+      }
     }
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
@@ -174,37 +189,39 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   }
 }
 
+// wide grids run the one-barrier variant
 template <class S, class ST, int COLL>
-TbFn pick_tb2_bar(int barriers) {
-  return barriers == 1 ? tb2_kernel<S, ST, COLL, 1> : tb2_kernel<S, ST, COLL, 2>;
+TbFn pick_tb2_bar(int barriers, bool wide) {
+  if (wide) return tb2_kernel<S, ST, COLL, 1, true>;
+  return barriers == 1 ? tb2_kernel<S, ST, COLL, 1, false> : tb2_kernel<S, ST, COLL, 2, false>;
 }
 
 template <class S, class ST>
-TbFn pick_tb2_coll(int coll, int barriers) {
+TbFn pick_tb2_coll(int coll, int barriers, bool wide) {
   switch (coll) {
-    case LBM_COLL_NONE: return pick_tb2_bar<S, ST, LBM_COLL_NONE>(barriers);
-    case LBM_COLL_BGK: return pick_tb2_bar<S, ST, LBM_COLL_BGK>(barriers);
-    case LBM_COLL_TRT: return pick_tb2_bar<S, ST, LBM_COLL_TRT>(barriers);
+    case LBM_COLL_NONE: return pick_tb2_bar<S, ST, LBM_COLL_NONE>(barriers, wide);
+    case LBM_COLL_BGK: return pick_tb2_bar<S, ST, LBM_COLL_BGK>(barriers, wide);
+    case LBM_COLL_TRT: return pick_tb2_bar<S, ST, LBM_COLL_TRT>(barriers, wide);
     case LBM_COLL_KBC:
-      if constexpr (S::ID == LBM_D3Q27) return pick_tb2_bar<S, ST, LBM_COLL_KBC>(barriers);
+      if constexpr (S::ID == LBM_D3Q27) return pick_tb2_bar<S, ST, LBM_COLL_KBC>(barriers, wide);
       else return nullptr;
   }
   return nullptr;
 }
 
 template <class S>
-TbFn pick_tb2(int storage, int coll, int barriers) {
+TbFn pick_tb2(int storage, int coll, int barriers, bool wide) {
   switch (storage) {
-    case LBM_F32: return pick_tb2_coll<S, StF32>(coll, barriers);
-    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, barriers);
-    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, barriers);
+    case LBM_F32: return pick_tb2_coll<S, StF32>(coll, barriers, wide);
+    case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, barriers, wide);
+    case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, barriers, wide);
   }
   return nullptr;
 }
 
-TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers) {
-  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll, barriers);
-  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll, barriers);
+TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers, bool wide) {
+  if (stencil == LBM_D3Q19) return pick_tb2<D3Q19>(storage, coll, barriers, wide);
+  if (stencil == LBM_D3Q27) return pick_tb2<D3Q27>(storage, coll, barriers, wide);
   return nullptr;
 }
 

@@ -257,7 +257,7 @@ __device__ __forceinline__ void cell_tables(const KParams& p, int64_t c, int (&a
   const int i2 = (int)(c % p.n2);
   const int64_t r = c / p.n2;
   const int i1 = (int)(r % p.n1), i0 = (int)(r / p.n1);
-  axis_offsets(i0, p.n0, p.s0, a0);
+  axis_offsets(i0, p.n0, (int)p.s0, a0);  // fewer than 2^31 populations
   axis_offsets(i1, p.n1, p.s1, a1);
   axis_offsets(i2, p.n2, 1, a2);
 }

@@ -165,17 +165,17 @@ class _TiledFlow(lt.ExtFlow):
             f[:, i * h:(i + 1) * h, j * h:(j + 1) * h].copy_(f[:, :h, :h])
 
 
-def test_largest_d3q19_grid_keeps_its_translation_symmetry():
+@pytest.mark.parametrize("layout", ["soa", "rows"])
+def test_largest_d3q19_grid_keeps_its_translation_symmetry(layout):
     """D3Q19 BGK FP32 at 1024^3 (1.07 G cells, two 76 GB population
     arrays): a state that repeats with period 512 along x and y keeps doing
     so bit for bit through 8 steps of the two-step kernel, since every cell
     runs the same arithmetic on the same neighbourhood.  This checks the
-    32-bit cell offsets near 2^30 and the tiling at the largest grid one
-    device holds; mass is conserved as well."""
+    64-bit plane offsets (2.0e10 elements in `rows`, whose planes interleave
+    all 19 populations), the 32-bit offsets inside a plane and the tiling at
+    a grid near the largest one device holds; mass is conserved as well."""
     n, h = 1024, 512
-    # SoA: the `rows` layout interleaves the populations of a row, so its
-    # cell offsets span all 19 arrays and exceed 32 bits at this size
-    flow = _TiledFlow(lt.Context("cuda:0", dtype=F32, layout="soa"), n, h)
+    flow = _TiledFlow(lt.Context("cuda:0", dtype=F32, layout=layout), n, h)
     sim = lt.Simulation(flow, lt.BGKCollision(0.51), [], lt.StreamingStrategy.PRE_STREAMING)
     assert sim._collide_and_stream.plan(8) == (4, 0)
     s0 = _sums(flow)

@@ -251,3 +251,45 @@ def test_halfway_classes_flags():
     near = (flags & N.CLASS_NEAR_SOLID) != 0
     assert int(near.sum()) == 26
     assert bool(flags[2, 2, 2] & N.CLASS_SOLID)
+
+
+def _plan_args(stencil, storage, n, stride, mask=False):
+    a = N.LbmStepArgs()
+    a.abi_version = N.ABI_VERSION
+    a.stencil, a.storage = stencil, storage
+    a.collision, a.tau, a.streaming = N.COLL_BGK, 0.6, 2
+    a.n[:], a.stride[:] = n, stride
+    a.f_in, a.f_out = 1 << 20, 1 << 40
+    if mask:
+        a.cell_class, a.classes, a.n_classes = 1 << 21, 1 << 22, 1
+    return a
+
+
+def test_grid_limits_apply_per_i0_plane_without_a_gpu():
+    """offsets inside the kernels are an i0 plane's 64-bit offset plus a
+    32-bit offset inside the plane: any number of planes, but a plane below
+    2^31 elements and (formats up to 4 bytes) 2^32 bytes; grids with
+    boundaries keep 32-bit cell offsets.  Checked before any HIP call."""
+    lib = N.lib()
+    plan = (ctypes.c_int64 * 2)()
+    Q, n = 19, 1024
+    # 1024^3 D3Q19 FP32 in `rows` ([X, Y, Q, Z] memory order): 2.0e10
+    # elements in all, 2.0e7 per plane
+    rows = _plan_args(N.D3Q19, N.F32, (n, n, n), (n, n * Q * n, Q * n, 1))
+    assert lib.lbm_run_plan(ctypes.byref(rows), 8, plan) == 0, lib.lbm_last_error()
+    assert list(plan) == [4, 0]
+    # a 40000^2 D2Q9 grid is one plane of 1.6e9 cells: above 2^30 FP32
+    # elements (2^32 bytes), below 2^31 16-bit elements
+    flat = (1, 40000, 40000)
+    flat_stride = (40000 * 40000, 40000 * 40000, 40000, 1)
+    big = _plan_args(N.D2Q9, N.F32, flat, flat_stride)
+    assert lib.lbm_run_plan(ctypes.byref(big), 1, plan) == -1
+    assert b"i0 plane" in lib.lbm_last_error()
+    half = _plan_args(N.D2Q9, N.BF16_SHIFTED, flat, flat_stride)
+    assert lib.lbm_run_plan(ctypes.byref(half), 1, plan) == 0, lib.lbm_last_error()
+    # with boundaries: at most 2^31 - 1 cells (here two planes of 1.2e9)
+    cells = (2, 40000, 30000)
+    masked = _plan_args(N.D3Q19, N.F16_SHIFTED, cells,
+                        (2 * 40000 * 30000, 40000 * 30000, 30000, 1), mask=True)
+    assert lib.lbm_run_plan(ctypes.byref(masked), 1, plan) == -1
+    assert b"2^31 - 1 cells" in lib.lbm_last_error()

@@ -148,3 +148,27 @@ def test_two_step_kernel_d3q27(coll, storage, layout, monkeypatch):
         # collision moves it further than in BGK
         tol = steps * ULPS[storage] * (50 if coll == "kbc" else 1)
         assert err <= tol, (steps, err)
+
+
+@pytest.mark.parametrize("stencil,coll,storage,layout", [
+    (O.D3Q19, "none", "f32", "soa"), (O.D3Q19, "bgk", "f32", "rows"),
+    (O.D3Q19, "bgk", "bf16_shifted", "rows"), (O.D3Q27, "kbc", "f32", "soa")])
+def test_wide_two_step_kernel_matches_the_narrow_one(stencil, coll, storage, layout, monkeypatch):
+    """the wide variant (each cell at a 64-bit plane offset plus a 32-bit
+    offset inside the plane, which grids of more than 2^31 elements use)
+    forced on a small grid, against the narrow variant (one 32-bit offset):
+    bit for bit without collision, else to the rounding of the collision"""
+    f0 = _f0([10, 16, 128], stencil=stencil)
+    make = {"bgk": lambda: lt.BGKCollision(0.62), "kbc": lambda: lt.KBCCollision(),
+            "none": lambda: lt.NoCollision()}[coll]
+    monkeypatch.setenv("LETTUCE_AMD_TB2", "2")
+    monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", "4")
+    out = {}
+    for wide in ("0", "1"):
+        monkeypatch.setenv("LETTUCE_AMD_WIDE", wide)
+        out[wide] = _run(f0, storage, layout, make(), 8, pairs=4)
+    if coll == "none":
+        assert np.array_equal(out["1"][0], out["0"][0])
+    else:
+        err = np.abs(out["1"][1] - out["0"][1]).max()
+        assert err <= 8 * ULPS[storage] * (50 if coll == "kbc" else 1), err
