@@ -142,7 +142,11 @@ typedef struct lbm_step_args {
   int32_t streaming;   /* enum lbm_streaming                                  */
   int32_t n_classes;   /* entries in `classes` (0 when cell_class == NULL)    */
   int64_t n[3];        /* extents (i0, i1, i2); 2D: n[0] == 1                 */
-  int64_t stride[4];   /* element strides of q, i0, i1, i2; stride[3] == 1    */
+  int64_t stride[4];   /* element strides of q, i0, i1, i2; stride[3] == 1.
+                        * One i0 plane ((n1-1)*stride[2] + n2 elements) must
+                        * stay below 2^31 elements and, for formats of up to
+                        * 4 bytes, 2^32 bytes; with cell_class the grid below
+                        * 2^31 cells.  Otherwise LBM_E_ARG.                   */
   const void* f_in;    /* populations before the step                         */
   void* f_out;         /* populations after the step (must not alias f_in)    */
   const uint8_t* cell_class;      /* [n0*n1*n2] class ids, or NULL (all collide, periodic) */
