@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
     };
     uint32_t nsm, flags, op;
     C f[Q];
-    gather_batched<S, ST, MASK, PULL, RAW>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
+    gather_batched<S, ST, MASK, PULL, RAW, true>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
 
     // non-equilibrium part and the MLP inputs
     C feat[6];

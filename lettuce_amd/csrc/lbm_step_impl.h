@@ -183,15 +183,25 @@ __device__ __forceinline__ void stage_classes(const KParams& p, lbm_cell_class* 
   for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
-// gather_pull / gather_own with the loads of a cell grouped into rounds:
-// with boundaries, first the class ids of the cell and of all its pull
-// sources (uint8 reads of the class map; their fields come from the LDS
-// table ctl), then every population, then the decodes.  Left alone, the
-// scheduler interleaves loads with their uses to save registers, which
-// serialises the memory round trips of these latency-bound kernels.
+// gather_pull / gather_own with the loads of a cell grouped into rounds.
+// With boundaries (MASK) and pull streaming there are two schedules:
+//  SPEC = false: first the class ids of the cell and of all its pull
+//    sources (uint8 reads of the class map, mostly L2 hits; the class
+//    fields come from the LDS table ctl), then every population;
+//  SPEC = true: every lane loads its plain pull sources together with its
+//    own class id; only lanes next to a solid (halfway bounce-back) or with
+//    no-stream bits then load the class ids of their sources and their own
+//    populations and replace the affected values.  A wave away from
+//    boundaries makes one round trip instead of two, a wave at a boundary
+//    two population round trips.  Measured on the 2048 x 512 D2Q9 channel:
+//    the learned kernel (long compute after the loads) gains 3-4 %, the
+//    step kernel (whose blocks wait for their wall wavefronts) loses 3-5 %.
+// Left alone, the scheduler interleaves loads with their uses to save
+// registers, which serialises the round trips of these latency-bound
+// kernels; the scheduling barriers keep each round together.
 // Returns the cell's no-stream bits, flags and op in nsm, flags, op.
 // RAW: f receives the stored values without the shift of shifted formats.
-template <class S, class ST, bool MASK, bool PULL, bool RAW = false, class X>
+template <class S, class ST, bool MASK, bool PULL, bool RAW = false, bool SPEC = false, class X>
 __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const X (&a0)[3],
                                                const int (&a1)[3], const int (&a2)[3],
                                                const lbm_cell_class* ctl, uint32_t& nsm,
@@ -208,15 +218,14 @@ __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const
     sinp[q] = PULL ? nb_in_plane<S>(q, -1, a1, a2) : a1[1] + a2[1];
     src[q] = MASK ? (int)(splane[q] + sinp[q]) : 0;
   }
-  uint32_t c0 = 0, cid[S::Q];
-  if (MASK) {
-    c0 = p.cls[cell];
-    if (PULL) {
+  constexpr bool TWO_ROUNDS = MASK && PULL && !SPEC;
+  uint32_t c0 = 0, cid0[S::Q];
+  if (MASK) c0 = p.cls[cell];
+  if constexpr (TWO_ROUNDS) {
 #pragma unroll
-      for (int q = 1; q < S::Q; ++q) cid[q] = p.cls[src[q]];
-    }
+    for (int q = 1; q < S::Q; ++q) cid0[q] = p.cls[src[q]];
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __builtin_amdgcn_sched_barrier(0);
   nsm = 0;
   flags = 0;
   op = LBM_OP_COLLIDE;
@@ -225,23 +234,57 @@ __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const
     flags = ctl[c0].flags;
     op = ctl[c0].op;
   }
-  // halfway bounce-back: a near-solid fluid cell whose pull source is solid
-  const bool near = MASK && PULL &&
-                    (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
   T raw[S::Q];
+  if constexpr (TWO_ROUNDS) {
+    const bool near =
+        (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
 #pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    int pop = q, idx = src[q];
-    if (MASK && PULL) {
-      if (near && q != 0 && (ctl[cid[q]].flags & LBM_CLASS_SOLID)) {
+    for (int q = 0; q < S::Q; ++q) {
+      int pop = q, idx = src[q];
+      if (near && q != 0 && (ctl[cid0[q]].flags & LBM_CLASS_SOLID)) {
         pop = S::opp[q];  // the cell's own opposite population
         idx = cell;
       } else if ((nsm >> q) & 1u) {
         idx = cell;
       }
+      raw[q] = fin[pop * p.sq + idx];
     }
-    raw[q] = (MASK && PULL) ? fin[pop * p.sq + idx]
-                            : elem(at_plane(fin + q * p.sq, splane[q]), in_plane(splane[q], sinp[q]));
+  } else {
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q)
+      raw[q] = elem(at_plane(fin + q * p.sq, splane[q]), in_plane(splane[q], sinp[q]));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (MASK && PULL && SPEC) {
+    // halfway bounce-back: a near-solid fluid cell whose pull source is
+    // solid takes its own opposite population; a no-stream bit keeps the
+    // cell's own value (source offsets are in the grid, so the first round
+    // read valid memory for these lanes too)
+    const bool near =
+        (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
+    if (near || nsm != 0) {
+      // D2Q9 / D3Q19: the own populations in the same round as the class
+      // ids; D3Q27 loads only those it needs, a round later (27 more
+      // registers would cost the learned kernel occupancy)
+      constexpr bool BATCH = S::Q <= 19;
+      uint32_t cid[S::Q];
+      T own[S::Q];
+#pragma unroll
+      for (int q = 1; q < S::Q; ++q) {
+        cid[q] = p.cls[src[q]];
+        if (BATCH) own[q] = fin[q * p.sq + cell];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 1; q < S::Q; ++q) {
+        if (near && (ctl[cid[q]].flags & LBM_CLASS_SOLID)) {
+          const int o = S::opp[q];
+          raw[q] = BATCH ? own[o] : fin[o * p.sq + cell];
+        } else if ((nsm >> q) & 1u) {
+          raw[q] = BATCH ? own[q] : fin[q * p.sq + cell];
+        }
+      }
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
