@@ -87,13 +87,15 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// Element i of a population array p, where p (the population's base plus
-// the offset of an i0 plane) is uniform across the wave and i is an offset
-// inside the plane.  For formats of up to 4 bytes the host keeps the byte
-// offsets inside a plane below 2^32, so p stays in scalar registers and
-// each access takes one 32-bit vector offset (the global_load saddr form)
-// instead of a 64-bit address computed per lane.  FP64 keeps 64-bit
-// addresses.  The accesses go through global (address space 1) pointers.
+// Element i of a population array p, where p (the population's base, plus
+// the offset of an i0 plane in the kernels' wide variants) is uniform
+// across the wave and i is a 32-bit element offset from it.  For formats of
+// up to 4 bytes the host keeps i * sizeof(T) below 2^32 (a plane's extent,
+// or in the narrow variants the whole array's), so p stays in scalar
+// registers and each access takes one 32-bit vector offset (the global_load
+// saddr form) instead of a 64-bit address computed per lane.  FP64 keeps
+// 64-bit addresses.  The accesses go through global (address space 1)
+// pointers.
 template <class T>
 using gptr = __attribute__((address_space(1))) T*;
 template <class T>
