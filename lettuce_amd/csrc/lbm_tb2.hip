@@ -155,6 +155,11 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 
     // ---- B: S2 of plane xb = x0 + it - 2 from S1 counters it-2 .. it ----
     if (it >= 2 && interior) {
+      // phase B at raised wave priority: the interior waves' step 2 and its
+      // stores go ahead of other waves' phase A arithmetic.  Measured on
+      // D3Q19 BGK 256^3 (two runs each): FP32 59.9k -> 60.7k MLUPS, BF16
+      // 97.9k -> 98.9k, FP16 unchanged (99.5k -> 99.9k).
+      __builtin_amdgcn_s_setprio(2);
       const int xb = x0 + it - 2;
       C g[S::Q];
 #pragma unroll
@@ -192,6 +197,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         else
           ST::st(fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
       }
+      __builtin_amdgcn_s_setprio(0);
     }
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
