@@ -97,6 +97,15 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   axis_offsets(y, p.n1, p.s1, ay);
   axis_offsets(z, p.n2, 1, az);
   const bool interior = !edge && row >= 1 && row <= TB_Y;
+  // The z-edge wave runs at raised priority throughout: phase B of the
+  // interior rows waits at the barrier for its 20 edge cells, and at the
+  // default priority it issues behind the 10 row waves.  Measured on D3Q19
+  // BGK 256^3 FP32 (two runs): 60.6k -> 63.8k MLUPS.
+  auto base_prio = [&] {
+    if (edge) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+  };
+  base_prio();
 
   const CollParams<C> cp = coll_params<C>(p);
   const T* fin = (const T*)p.fin;
@@ -132,7 +141,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
           raw[q] = elem(fin + q * p.sq, ax[1 - go<S>(q, 0)] + in);
       }
       __builtin_amdgcn_sched_barrier(0);
-      if (PRIO) __builtin_amdgcn_s_setprio(0);
+      if (PRIO) base_prio();
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) cur[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
       collide<S, C, COLL>(cur, cp);
@@ -197,7 +206,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         else
           ST::st(fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
       }
-      __builtin_amdgcn_s_setprio(0);
+      base_prio();
     }
 #pragma unroll
     for (int q = 0; q < S::Q; ++q) {
