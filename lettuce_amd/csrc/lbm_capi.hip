@@ -181,10 +181,12 @@ static int block_order(const KParams& p, int64_t nb) {
 
 // the two-step kernel for this step, if one applies: periodic D3Q19 or
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
-// TB_Y.  Used by default for D3Q19, with one barrier per plane (measured on
-// MI355X, BGK TGV 256^3: FP32 38.6k -> 58.9k MLUPS, FP16-shifted 92.8k; with
-// two barriers per plane FP16 was 5 % slower).  D3Q27 KBC loses (27.1k ->
-// 25.1k: 125-143 VGPRs leave one 11-wavefront block per CU).
+// TB_Y.  Used by default for D3Q19 in every format and for D3Q27 in FP32,
+// with one barrier per plane.  Measured on MI355X, TGV 256^3: D3Q19 BGK FP32
+// 38.3k (single steps) -> 63.6k MLUPS, FP16-shifted 100k; D3Q27 KBC FP32
+// 26.5k -> 29.4k, but D3Q27 KBC BF16 46.4k -> 42.8k (its arithmetic, not
+// its memory traffic, bounds it, and the two-step kernel recomputes the
+// halo rows), so 16-bit D3Q27 stays on single steps.
 // LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it applies;
 // LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64),
 // LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=2
@@ -195,7 +197,8 @@ static int block_order(const KParams& p, int64_t nb) {
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = env_int("LETTUCE_AMD_TB2", 1);
   if (mode == 0) return nullptr;
-  if (mode == 1 && a->stencil != LBM_D3Q19) return nullptr;
+  if (mode == 1 && !(a->stencil == LBM_D3Q19 || (a->stencil == LBM_D3Q27 && a->storage == LBM_F32)))
+    return nullptr;
   if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;

@@ -107,7 +107,7 @@ def test_two_step_kernel_matches_the_oracle(layout, monkeypatch):
 
 
 def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
-    """D3Q19 in every storage format by default; not D3Q27 by default; no
+    """D3Q19 in every storage format by default, D3Q27 in FP32 only; no
     two-step launch with push streaming or extents that do not tile"""
     monkeypatch.delenv("LETTUCE_AMD_TB2", raising=False)
     f0 = _f0([4, 16, 64])
@@ -126,6 +126,10 @@ def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
     assert sim._collide_and_stream.plan(8) == (0, 8)
     f2 = _f0([4, 16, 64], stencil=O.D3Q27)
     sim = lt.Simulation(_Flow(c, f2.astype(np.float32)), lt.KBCCollision(), [],
+                        lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (4, 0)
+    c16 = lt.Context("cuda:0", dtype=F32, storage="bf16_shifted")
+    sim = lt.Simulation(_Flow(c16, f2.astype(np.float32)), lt.KBCCollision(), [],
                         lt.StreamingStrategy.PRE_STREAMING)
     assert sim._collide_and_stream.plan(8) == (0, 8)
 
