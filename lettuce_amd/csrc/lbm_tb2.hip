@@ -97,11 +97,16 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   axis_offsets(y, p.n1, p.s1, ay);
   axis_offsets(z, p.n2, 1, az);
   const bool interior = !edge && row >= 1 && row <= TB_Y;
+  // D3Q19 (two blocks per CU) uses wave priorities; D3Q27, whose registers
+  // leave one block per CU, runs faster without them (KBC FP32 256^3:
+  // 29.4k with, 30.8k MLUPS without).
+  constexpr bool PRIORITIES = S::Q == 19;
   // The z-edge wave runs at raised priority throughout: phase B of the
   // interior rows waits at the barrier for its 20 edge cells, and at the
   // default priority it issues behind the 10 row waves.  Measured on D3Q19
   // BGK 256^3 FP32 (two runs): 60.6k -> 63.8k MLUPS.
   auto base_prio = [&] {
+    if (!PRIORITIES) return;
     if (edge) __builtin_amdgcn_s_setprio(1);
     else __builtin_amdgcn_s_setprio(0);
   };
@@ -128,7 +133,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
       // D3Q19 BGK 256^3: FP16 89.7k -> 94.4k MLUPS from the batching and
       // -> 98.3k with the priority; FP32 59.6k -> 59.8k from the batching
       // but 58.2k with the priority, so FP32 runs without it.
-      constexpr bool PRIO = sizeof(T) == 2;
+      constexpr bool PRIO = PRIORITIES && sizeof(T) == 2;
       if (PRIO) __builtin_amdgcn_s_setprio(3);
       __builtin_amdgcn_sched_barrier(0);
       T raw[S::Q];
@@ -168,7 +173,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
       // stores go ahead of other waves' phase A arithmetic.  Measured on
       // D3Q19 BGK 256^3 (two runs each): FP32 59.9k -> 60.7k MLUPS, BF16
       // 97.9k -> 98.9k, FP16 unchanged (99.5k -> 99.9k).
-      __builtin_amdgcn_s_setprio(2);
+      if (PRIORITIES) __builtin_amdgcn_s_setprio(2);
       const int xb = x0 + it - 2;
       C g[S::Q];
 #pragma unroll
