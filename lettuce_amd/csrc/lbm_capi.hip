@@ -189,8 +189,8 @@ static int block_order(const KParams& p, int64_t nb) {
 // halo rows), so 16-bit D3Q27 stays on single steps.
 // LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it applies;
 // LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64),
-// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=2
-// the variant with two barriers per plane and shallower LDS rings.  A
+// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=1 or 2
+// the variant with one barrier per plane or two with shallower LDS rings.  A
 // variant that loaded the next plane during phase B measured slower
 // (53.0k vs 57.8k MLUPS: its 116 VGPRs left one block per CU) and was
 // removed.
@@ -202,8 +202,12 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   if (a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->n[2] % 64 != 0 || a->n[1] % TB_Y != 0) return nullptr;
   if (a->n[2] * a->n[1] * a->n[0] > (int64_t)0x7fffffff) return nullptr;
-  return pick_tb2_kernel(a->stencil, a->storage, a->collision,
-                         env_int("LETTUCE_AMD_TB2_BARRIERS", 1) == 2 ? 2 : 1,
+  // one barrier per plane for FP32, two (with the shallower LDS rings) for
+  // the 16-bit formats: measured on D3Q19 BGK 256^3 with the wave
+  // priorities, FP16 99.3k -> 102.5k MLUPS and BF16 99.6k -> 102.7k with
+  // two, FP32 63.5k -> 63.1k
+  const int barriers = env_int("LETTUCE_AMD_TB2_BARRIERS", storage_bytes(a->storage) == 2 ? 2 : 1);
+  return pick_tb2_kernel(a->stencil, a->storage, a->collision, barriers == 2 ? 2 : 1,
                          wide_offsets(a->storage, a->n, a->stride));
 }
 
