@@ -56,7 +56,7 @@ enum lbm_storage {
  *  KBC     lettuce/ext/_collision/kbc_collision.py:96-160 (D2Q9, D3Q27)
  *  LEARNED KBC decomposition whose higher-order rates come from a per-cell
  *          MLP (no lettuce counterpart; nearest hook MRTCollision,
- *          lettuce/ext/_collision/mrt_collision.py:8-35), D2Q9 only.
+ *          lettuce/ext/_collision/mrt_collision.py:8-35), D2Q9 and D3Q27.
  *  BGK_GUO BGK with Guo forcing by a uniform acceleration a
  *          (bgk_collision.py:17-22 with ext/_force/guo.py:15-36):
  *          u* = j / rho + a / (2 rho),
@@ -113,12 +113,15 @@ typedef struct lbm_cell_class {
   double u[3];        /* EQUILIBRIUM velocity, lattice components (x, y, z)    */
 } lbm_cell_class;
 
-/* Learned collision (D2Q9).  With df = f - feq, ds the KBC shear part of df
- * and dh = df - ds (which holds only the raw moments xxy, xyy, xxyy):
+/* Learned collision.  With df = f - feq, ds the KBC shear part of df and
+ * dh = df - ds (D2Q9: dh holds only the raw moments xxy, xyy, xxyy):
  *   x   = (T, N, Pxy, xxy, xyy, xxyy)(df) / rho * in_scale        (bf16)
+ *         (D3Q27: T, Nxz, Nyz, Pxy, Pxz, Pyz of df)
  *   o   = W3 silu(W2 silu(W1 x + b1) + b2) + b3    (bf16 MFMA, FP32 sums)
  *   r_g = 1/tau + o_g, clamped into [rate_lo, rate_hi]
  *   f'  = f - ds/tau - sum_g r_g m_g(dh) phi_g
+ * (D2Q9: the groups are the moments xxy, xyy, xxyy; D3Q27: the raw moments
+ * x^a y^b z^c of order 3, of order 4, and of orders 5 and 6).
  * A cell whose raw r_g is NaN or outside [stable_lo, stable_hi] takes the
  * KBC update instead.  `packed` is the device copy of the
  * LBM_LEARNED_PACKED_BYTES buffer written by lbm_learned_pack. */
@@ -186,7 +189,8 @@ int lbm_run(const lbm_step_args* args, int64_t nsteps, void* stream);
 /* How lbm_run would execute `nsteps` steps, without launching anything:
  * plan[0] = launches of the two-step kernel (two fused steps each: periodic
  * D3Q19 / D3Q27 without boundaries, pull streaming, contiguous extent a
- * multiple of 64, i1 extent a multiple of 8; by default for D3Q19),
+ * multiple of 64, i1 extent a multiple of 8; by default for D3Q19 in every
+ * storage format and for D3Q27 in FP32),
  * plan[1] = single-step launches.  No lettuce counterpart; for reporting. */
 int lbm_run_plan(const lbm_step_args* args, int64_t nsteps, int64_t* plan);
 
