@@ -280,6 +280,11 @@ class Flow(ABC):
             if list(z["resolution"]) != list(self.resolution):
                 raise ValueError(f"checkpoint resolution {list(z['resolution'])} differs "
                                  f"from {self.resolution}")
+            # shifted 16-bit states decode against rho0_shift: restoring the
+            # bits under another shift would silently change every population
+            if self.context.shifted and float(z["rho0_shift"]) != float(self.rho0_shift):
+                raise ValueError(f"checkpoint rho0_shift {float(z['rho0_shift'])} differs "
+                                 f"from this flow's {self.rho0_shift}")
             if check_units:
                 for k in self._UNIT_FIELDS:
                     if float(z["unit_" + k]) != float(getattr(self.units, k)):

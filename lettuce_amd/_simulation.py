@@ -100,6 +100,8 @@ class NativeStep:
         a.force_tau = desc.force_tau
         a.accel[:] = list(desc.accel) + [0.0] * (3 - len(desc.accel))
         a.rho0_shift = flow.rho0_shift
+        # the descriptor owns the learned struct and its packed weights; it
+        # stays alive with this step (self.collision_desc)
         self._learned = desc.learned_struct()
         if self._learned is not None:
             a.learned = ctypes.pointer(self._learned)

@@ -137,9 +137,11 @@ class HighMaReporter(FailureReporterBase):
         self.name = "HighMa"
 
     def is_failed(self, simulation) -> bool:
-        nan_cells, _, uu_max = _scan(simulation.flow).tolist()
+        nan_pops, bad_u, uu_max = _scan(simulation.flow).tolist()
         cs = simulation.flow.stencil.cs
-        return nan_cells > 0 or uu_max ** 0.5 / cs > self.threshold
+        # bad_u counts cells whose |u|^2 is not finite (an inf population, a
+        # zero density), which need not hold a NaN population
+        return bool(nan_pops > 0 or bad_u > 0 or uu_max ** 0.5 / cs > self.threshold)
 
     def get_results(self, simulation):
         flow = simulation.flow

@@ -83,8 +83,6 @@ class LearnedCollision(Collision):
 
     def __init__(self, params: Optional[LearnedParameters] = None):
         self.params = params or LearnedParameters.seeded(0)
-        self._packed = None
-        self._struct = None
 
     def __call__(self, flow):
         return apply_collision(self, flow)
@@ -99,13 +97,14 @@ class LearnedCollision(Collision):
         if not isinstance(flow.stencil, (D2Q9, D3Q27)):
             raise NotImplementedError("the learned collision is defined for D2Q9 and D3Q27")
         desc.tau = flow.units.relaxation_parameter_lu
-        self._packed = self.params.packed(flow.context.device)
+        # one packed weight buffer per binding, owned by the descriptor: a
+        # second binding of this collision (another Simulation, or
+        # collision(flow)) gets its own and leaves this one in place
+        packed = self.params.packed(flow.context.device)
         s = N.LbmLearned()
-        s.packed = self._packed.data_ptr()
+        s.packed = packed.data_ptr()
         p = self.params
         s.in_scale, s.rate_lo, s.rate_hi = p.in_scale, p.rate_lo, p.rate_hi
         s.stable_lo, s.stable_hi = p.stable_lo, p.stable_hi
-        self._struct = s
-
-    def _learned_struct(self):
-        return self._struct
+        desc.learned = s
+        desc.keep = [packed]

@@ -74,6 +74,11 @@ class NativeCollisionDesc:
     owner: object = None
     force_tau: float = 0.0                  # BGK_GUO
     accel: Tuple[float, ...] = (0.0, 0.0, 0.0)
+    # LEARNED: the lbm_learned struct of this binding and the device tensors
+    # it points into; they live as long as the descriptor, which the native
+    # step holds, so binding the same collision again cannot free them
+    learned: object = None
+    keep: List[torch.Tensor] = field(default_factory=list)
 
     def bind(self, flow):
         if self.kind == N.COLL_KBC:
@@ -87,9 +92,7 @@ class NativeCollisionDesc:
             self.owner._bind(self, flow)
 
     def learned_struct(self):
-        if self.owner is not None and hasattr(self.owner, "_learned_struct"):
-            return self.owner._learned_struct()
-        return None
+        return self.learned
 
 
 def apply_collision(collision, flow) -> torch.Tensor:
@@ -411,20 +414,24 @@ class HalfwayBounceBackBoundary(Boundary):
         over links from fluid x into the solid (halfway_bounce_back_boundary.
         py:211-218), from the post-collision populations of the last step."""
         flow = simulation.flow
-        links = getattr(self, "_links", None)
-        if links is None:
+        post = simulation.streaming_strategy.post_streaming()
+        # the link list depends on the flow (stencil, grid, device) and the
+        # streaming order, so it is cached per (flow, order): one boundary
+        # object may serve several simulations
+        key = (id(flow), type(flow.stencil).__name__, tuple(flow.resolution),
+               str(flow.context.device), post)
+        cache = self.__dict__.setdefault("_link_cache", {})
+        if key not in cache:
             from .._classes import halfway_links
             links = halfway_links(self.mask, flow.stencil.e, flow.stencil.opposite)
-            if simulation.streaming_strategy.post_streaming():
+            if post:
                 # after a push step f*_d(x) sits in slot opp(d) of x
                 opp = torch.tensor(flow.stencil.opposite, dtype=torch.int32)
                 links = torch.stack([links[:, 0], opp[links[:, 1].long()]], 1)
-                self._sign = -1.0
-            else:
-                self._sign = 1.0
-            self._links = links = links.to(flow.context.device)
+            cache[key] = (links.to(flow.context.device), -1.0 if post else 1.0)
+        links, sign = cache[key]
         F = _ops.link_force_f64(flow.stencil, flow._state, links, flow.rho0_shift)
-        self.force_sum = F[:flow.stencil.d] * self._sign
+        self.force_sum = F[:flow.stencil.d] * sign
         return self.force_sum
 
 

@@ -56,6 +56,27 @@ def test_restart_is_bit_exact(tmp_path, storage, layout, d):
     assert torch.equal(_bits(b._state), _bits(a._state))
 
 
+def test_checkpoint_rho0_shift_mismatch_is_rejected(tmp_path):
+    """shifted storage decodes against rho0_shift: a file written under one
+    shift must not load into a flow that decodes with another"""
+    a = _flow("f16_shifted")
+    path = tmp_path / "ck.npz"
+    a.save_checkpoint(path)
+
+    class Shifted(lt.TaylorGreenVortex):
+        @property
+        def rho0_shift(self):
+            return 0.5
+
+    ctx = lt.Context("cuda:0", dtype=torch.float32, storage="f16_shifted")
+    b = Shifted(ctx, 32, 100, 0.1, lt.D2Q9(), initialize_fneq=False)
+    with pytest.raises(ValueError, match="rho0_shift"):
+        b.load_checkpoint(path)
+    c = _flow("f16_shifted")
+    c.load_checkpoint(path)
+    assert torch.equal(_bits(c._state), _bits(a._state))
+
+
 def test_checkpoint_mismatches_are_rejected(tmp_path):
     a = _flow("f32")
     path = tmp_path / "ck.npz"

@@ -90,3 +90,15 @@ def test_high_ma_reporter_threshold():
     top = rep.results[0][1]
     assert abs(top - ma.max().item()) <= 1e-6 and len(rep.results) == 3
     assert rep.results[0][1] >= rep.results[1][1] >= rep.results[2][1]
+
+
+def test_high_ma_reporter_fails_on_inf_population_without_nan():
+    """an overflowing population (inf, no NaN) leaves |u|^2 non-finite in its
+    cell: HighMaReporter must stop there although no population is NaN"""
+    flow = _tgv(res=16)
+    state = flow._state
+    state[1, 3, 4] = float("inf")
+    out = _ops.scan_f64(flow.stencil, state).cpu()
+    assert out[0].item() == 0 and out[1].item() == 1
+    rep = lt.HighMaReporter(1, threshold=10.0, k=2)
+    assert rep.is_failed(_sim(flow, []))

@@ -203,14 +203,14 @@ def test_d3q27_learned_with_bounce_back_vs_oracle(strategy, storage):
 
 
 @pytest.mark.parametrize("case", ["d3q27_bf16", "d3q27_f32", "d2q9_channel_f16"])
-def test_wide_learned_kernel_matches_the_narrow_one(case, monkeypatch):
+def test_wide_learned_kernel_matches_the_narrow_one(case, knobs):
     """the wide learned kernel (64-bit plane offsets, which grids of more
     than 2^31 elements use) forced on a small grid, against the narrow one:
     equal to the rounding of the collision arithmetic"""
     p, _ = pair(0, out_scale=0.1)
     out = {}
     for wide in ("0", "1"):
-        monkeypatch.setenv("LETTUCE_AMD_WIDE", wide)
+        knobs(wide_offsets=wide)
         if case.startswith("d3q27"):
             storage = "f32" if case.endswith("f32") else "bf16_shifted"
             flow = ArrayFlow(ctx(F32, storage), _f27(seed=8, amp=0.01), lt.D3Q27())
@@ -228,3 +228,29 @@ def test_wide_learned_kernel_matches_the_narrow_one(case, monkeypatch):
     err = rel_err(out["1"], out["0"])
     print(f"wide vs narrow learned ({case}): {err:.3e}")
     assert err <= (1e-6 if case == "d3q27_f32" else 1e-3), err
+
+
+def test_rebinding_a_learned_collision_keeps_the_first_binding_alive():
+    """A second binding of one LearnedCollision (collision(flow), another
+    Simulation) must not free the weight buffer the first Simulation's
+    kernels read: after the rebind and fresh allocations that could reuse
+    freed memory, the first simulation steps bit for bit like one built from
+    a fresh collision."""
+    p, _ = pair(0, out_scale=0.5)
+
+    def tgv():
+        return lt.TaylorGreenVortex(ctx(F32), 64, 300, 0.1, lt.D2Q9(), initialize_fneq=False)
+
+    coll = lt.LearnedCollision(p)
+    a = tgv()
+    sim_a = lt.Simulation(a, coll, [], lt.StreamingStrategy.PRE_STREAMING)
+    coll(tgv())                                  # second binding, then dropped
+    lt.Simulation(tgv(), coll, [], lt.StreamingStrategy.PRE_STREAMING)
+    torch.cuda.synchronize()
+    junk = [torch.full((17408 // 4,), float(k + 7), device="cuda:0") for k in range(64)]
+    sim_a(20)
+    b = tgv()
+    lt.Simulation(b, lt.LearnedCollision(lt.LearnedParameters.seeded(0, out_scale=0.5)), [],
+                  lt.StreamingStrategy.PRE_STREAMING)(20)
+    assert torch.equal(a._state.cpu(), b._state.cpu())
+    del junk
