@@ -183,16 +183,32 @@ int lbm_step(const lbm_step_args* args, void* stream);
  * written by step k-1).  The result is in f_out when nsteps is odd and in
  * f_in when it is even; f_in is overwritten when nsteps >= 2.  Replaces the
  * per-step Python loop of Simulation.__call__ (lettuce/_simulation.py:317-320)
- * when no reporter runs in between. */
+ * when no reporter runs in between.  A post-streaming run (lettuce's default
+ * order, collide then stream, _simulation.py:54, :154-157) of nsteps >= 2
+ * without the learned collision runs as (S C S) (C S)^(nsteps-2) C: one
+ * collide-only launch, nsteps - 2 pull steps, one double-streaming launch;
+ * the composition is the same, so every cell takes the same values as with
+ * nsteps push steps. */
 int lbm_run(const lbm_step_args* args, int64_t nsteps, void* stream);
 
 /* How lbm_run would execute `nsteps` steps, without launching anything:
  * plan[0] = launches of the two-step kernel (two fused steps each: periodic
- * D3Q19 / D3Q27 without boundaries, pull streaming, contiguous extent a
- * multiple of 64, i1 extent a multiple of 8; by default for D3Q19 in every
- * storage format and for D3Q27 in FP32),
- * plan[1] = single-step launches.  No lettuce counterpart; for reporting. */
+ * D3Q19 / D3Q27 without boundaries, pull streaming (or the pull steps of a
+ * split post-streaming run), contiguous extent a multiple of 64, i1 extent a
+ * multiple of 8; by default for D3Q19 in every storage format and for D3Q27
+ * in FP32), plan[1] = single-step launches.  No lettuce counterpart; for
+ * reporting. */
 int lbm_run_plan(const lbm_step_args* args, int64_t nsteps, int64_t* plan);
+
+/* Launch tuning for measurements and tests (no lettuce counterpart): set the
+ * process-wide knob `name` to `value`, or back to the value its environment
+ * variable gave at load when value < 0.  Knobs: "two_step" (0 never, 1
+ * default selection, 2 wherever the two-step kernel applies),
+ * "two_step_planes" (x chunk, 0 = per grid), "two_step_barriers" (1 or 2,
+ * 0 = per format), "wide_offsets" (1 forces 64-bit plane offsets),
+ * "post_split" (0 runs post-streaming with the push kernel only).  Not
+ * synchronised with launches on other threads. */
+int lbm_set_tuning(const char* name, int32_t value);
 
 /* f[q][cell] = feq_q(rho[cell], u[:, cell]) written in the given storage:
  * QuadraticEquilibrium (lettuce/ext/_equilibrium/quadratic_equilibrium.py:11-24)

@@ -16,7 +16,7 @@ import threading
 import torch
 
 __all__ = ["lib", "LbmStepArgs", "LbmCellClass", "LbmLearned", "check",
-           "stream_handle", "STENCIL_IDS", "STORAGE_IDS", "LIB_PATH",
+           "stream_handle", "tuning", "STENCIL_IDS", "STORAGE_IDS", "LIB_PATH",
            "ABI_VERSION"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -108,6 +108,7 @@ _SIGNATURES = {
     "lbm_halo_pack": ([_i32, _i32, _I64_3, _I64_4, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_slab_step": ([ctypes.POINTER(LbmStepArgs), _i32, _vp, _vp, _vp, _vp, _vp],
                       ctypes.c_int),
+    "lbm_set_tuning": ([ctypes.c_char_p, _i32], ctypes.c_int),
 }
 
 _lock = threading.Lock()
@@ -149,6 +150,26 @@ def check(rc, what):
     if rc != 0:
         msg = lib().lbm_last_error().decode(errors="replace")
         raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+class tuning:
+    """Context manager that sets launch-tuning knobs of the library
+    (lbm_set_tuning: two_step, two_step_planes, two_step_barriers,
+    wide_offsets, post_split) and restores their start values on exit.
+    The knobs are process-wide; they exist for measurements and tests."""
+
+    def __init__(self, **knobs):
+        self.knobs = knobs
+
+    def __enter__(self):
+        for name, value in self.knobs.items():
+            check(lib().lbm_set_tuning(name.encode(), int(value)), "lbm_set_tuning")
+        return self
+
+    def __exit__(self, *exc):
+        for name in self.knobs:
+            lib().lbm_set_tuning(name.encode(), -1)
+        return False
 
 
 def stream_handle(device):

@@ -22,26 +22,47 @@ KernelFn pick_kernel_d3q27(int storage, int coll, bool mask, int mode);
 int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int storage, bool mask,
                    int mode, bool wide, hipStream_t stream, int64_t nsteps, const void* f_in,
                    void* f_out);
-KernelFn pick_pair_kernel_d2q9(int storage, int coll, int mode, bool nt);
-KernelFn pick_pair_kernel_d3q19(int storage, int coll, int mode, bool nt);
-KernelFn pick_pair_kernel_d3q27(int storage, int coll, int mode, bool nt);
-KernelFn pick_row_kernel_d2q9(int storage, int coll);
-KernelFn pick_row_kernel_d3q19(int storage, int coll);
-KernelFn pick_row_kernel_d3q27(int storage, int coll);
 }  // namespace lbm
 
-// Launch tuning read from the environment (for measurements):
-//  LETTUCE_AMD_CELLS_PER_THREAD=1|2  pair kernel where available (default 1:
-//                                    measured slower on MI355X, 5.30 vs 5.68 TB/s)
-//  LETTUCE_AMD_NT_STORE=0|1          non-temporal stores in the pair kernel (default 1)
-//  LETTUCE_AMD_ROW_KERNEL=0|1        one block per row, aligned loads and lane
-//                                    shifts, where rows are 64..1024 long (default 0:
-//                                    measured equal for FP32 BGK, slower for FP16
-//                                    storage and KBC)
-static int env_int(const char* name, int dflt) {
-  const char* v = getenv(name);
-  return v ? atoi(v) : dflt;
-}
+// Launch tuning, for measurements and tests.  Each knob starts from its
+// environment variable, read once when the library is loaded, and
+// lbm_set_tuning changes it at run time (value < 0 restores the start value):
+//  two_step           LETTUCE_AMD_TB2           0 never, 1 the default
+//                     selection (see two_step_kernel), 2 wherever it applies
+//  two_step_planes    LETTUCE_AMD_TB2_PLANES    x chunk of the two-step kernel
+//                                               (0: chosen per grid)
+//  two_step_barriers  LETTUCE_AMD_TB2_BARRIERS  1 or 2 barriers per plane
+//                                               (0: chosen per format)
+//  wide_offsets       LETTUCE_AMD_WIDE          1 forces the 64-bit i0 plane
+//                                               offsets (0: chosen per grid)
+//  post_split         LETTUCE_AMD_POST_SPLIT    0 runs post-streaming with the
+//                                               push kernel only (see lbm_run)
+namespace {
+struct Knob {
+  const char* name;
+  const char* env;
+  int dflt;
+  int start;
+  int value;
+};
+Knob g_knobs[] = {
+    {"two_step", "LETTUCE_AMD_TB2", 1, 0, 0},
+    {"two_step_planes", "LETTUCE_AMD_TB2_PLANES", 0, 0, 0},
+    {"two_step_barriers", "LETTUCE_AMD_TB2_BARRIERS", 0, 0, 0},
+    {"wide_offsets", "LETTUCE_AMD_WIDE", 0, 0, 0},
+    {"post_split", "LETTUCE_AMD_POST_SPLIT", 1, 0, 0},
+};
+enum KnobId { K_TWO_STEP, K_TB2_PLANES, K_TB2_BARRIERS, K_WIDE, K_POST_SPLIT, K_COUNT };
+struct KnobInit {
+  KnobInit() {
+    for (Knob& k : g_knobs) {
+      const char* v = getenv(k.env);
+      k.start = k.value = v ? atoi(v) : k.dflt;
+    }
+  }
+} g_knob_init;
+inline int knob(KnobId id) { return g_knobs[id].value; }
+}  // namespace
 
 using namespace lbm;
 
@@ -137,10 +158,10 @@ static int check_layout(int stencil, int storage, const int64_t n[3], const int6
 // Whether a layout needs the wide (64-bit i0 plane) offsets of the
 // two-step and learned kernels: their narrow variants address a cell by one
 // 32-bit offset, in bytes for formats of up to 4 bytes.
-// LETTUCE_AMD_WIDE=1 selects the wide variants for every grid (tests run
-// them on small grids against the narrow ones).
+// The knob wide_offsets = 1 selects the wide variants for every grid (tests
+// run them on small grids against the narrow ones).
 static bool wide_offsets(int storage, const int64_t n[3], const int64_t stride[4]) {
-  if (env_int("LETTUCE_AMD_WIDE", 0) == 1) return true;
+  if (knob(K_WIDE) == 1) return true;
   const int64_t span = (n[0] - 1) * stride[1] + (n[1] - 1) * stride[2] + n[2];
   const int bytes = storage_bytes(storage);
   const int64_t limit = bytes <= 4 ? std::min<int64_t>((int64_t)1 << 31, ((int64_t)1 << 32) / bytes)
@@ -163,22 +184,6 @@ static int block_size_for(int64_t n2) {
   return 256;
 }
 
-// Block order for a grid of nb blocks: storage order unless
-// LETTUCE_AMD_BLOCK_ORDER=slab|chunk asks for an XCD-aware order (and the
-// grid splits evenly over the 8 XCDs).
-static int block_order(const KParams& p, int64_t nb) {
-  const bool slab_ok = nb % 8 == 0 && nb >= 64;
-  const bool chunk_ok = slab_ok && p.n1 % 8 == 0;
-  const char* env = getenv("LETTUCE_AMD_BLOCK_ORDER");
-  if (env) {
-    if (!strcmp(env, "slab") && slab_ok) return ORDER_XCD_SLAB;
-    if (!strcmp(env, "chunk") && chunk_ok) return ORDER_XCD_CHUNK;
-  }
-  // measured on MI355X (D3Q19 256^3 FP32): storage order 5.69 TB/s, XCD
-  // chunks 5.57 TB/s, XCD slabs 4.76 TB/s
-  return ORDER_LINEAR;
-}
-
 // the two-step kernel for this step, if one applies: periodic D3Q19 or
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
 // TB_Y.  Used by default for D3Q19 in every format and for D3Q27 in FP32,
@@ -187,15 +192,14 @@ static int block_order(const KParams& p, int64_t nb) {
 // 26.5k -> 29.4k, but D3Q27 KBC BF16 46.4k -> 42.8k (its arithmetic, not
 // its memory traffic, bounds it, and the two-step kernel recomputes the
 // halo rows), so 16-bit D3Q27 stays on single steps.
-// LETTUCE_AMD_TB2=0 turns it off, =2 uses it wherever it applies;
-// LETTUCE_AMD_TB2_PLANES sets the x chunk (default 64),
-// LETTUCE_AMD_TB2_XCD=0 the plain tile order, LETTUCE_AMD_TB2_BARRIERS=1 or 2
-// the variant with one barrier per plane or two with shallower LDS rings.  A
-// variant that loaded the next plane during phase B measured slower
-// (53.0k vs 57.8k MLUPS: its 116 VGPRs left one block per CU) and was
-// removed.
+// The knob two_step = 0 turns it off, = 2 uses it wherever it applies;
+// two_step_planes sets the x chunk, two_step_barriers = 1 or 2 the variant
+// with one barrier per plane or two with shallower LDS rings.  A variant
+// that loaded the next plane during phase B measured slower (53.0k vs 57.8k
+// MLUPS: its 116 VGPRs left one block per CU) and was removed, and so was
+// the plain tile order (XCD order: 54.7k -> 57.8k MLUPS).
 static TbFn two_step_kernel(const lbm_step_args* a) {
-  const int mode = env_int("LETTUCE_AMD_TB2", 1);
+  const int mode = knob(K_TWO_STEP);
   if (mode == 0) return nullptr;
   if (mode == 1 && !(a->stencil == LBM_D3Q19 || (a->stencil == LBM_D3Q27 && a->storage == LBM_F32)))
     return nullptr;
@@ -206,7 +210,8 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   // the 16-bit formats: measured on D3Q19 BGK 256^3 with the wave
   // priorities, FP16 99.3k -> 102.5k MLUPS and BF16 99.6k -> 102.7k with
   // two, FP32 63.5k -> 63.1k
-  const int barriers = env_int("LETTUCE_AMD_TB2_BARRIERS", storage_bytes(a->storage) == 2 ? 2 : 1);
+  const int barriers =
+      knob(K_TB2_BARRIERS) > 0 ? knob(K_TB2_BARRIERS) : (storage_bytes(a->storage) == 2 ? 2 : 1);
   return pick_tb2_kernel(a->stencil, a->storage, a->collision, barriers == 2 ? 2 : 1,
                          wide_offsets(a->storage, a->n, a->stride));
 }
@@ -220,10 +225,9 @@ static int64_t two_step_pairs(TbFn tb, int64_t nsteps) {
   return pairs;
 }
 
-// generic: use step_kernel (one cell per thread, block_tile indexing) even
-// where the pair or row kernels are requested
-static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block,
-                        bool generic = false) {
+// Validates the arguments and prepares the single-step kernel (one cell per
+// thread, block_tile indexing) of a step with a->streaming.
+static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& grid, dim3& block) {
   if (!a) return fail(LBM_E_ARG, "NULL args");
   if (a->abi_version != LBM_ABI_VERSION)
     return fail(LBM_E_VERSION, "abi_version %d != library %d", a->abi_version, LBM_ABI_VERSION);
@@ -294,43 +298,11 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.s1 = (int)a->stride[2];
   p.sq = a->stride[0];
   p.ncells = a->n[0] * a->n[1] * a->n[2];
-  // two cells per thread where requested, a pair kernel exists and rows
-  // have even length
-  int64_t row_threads = a->n[2];
-  if (fn && !generic && !mask && a->n[2] % 2 == 0 && env_int("LETTUCE_AMD_CELLS_PER_THREAD", 1) == 2) {
-    const bool nt = env_int("LETTUCE_AMD_NT_STORE", 1) != 0;
-    KernelFn pf = nullptr;
-    switch (a->stencil) {
-      case LBM_D2Q9: pf = pick_pair_kernel_d2q9(a->storage, a->collision, a->streaming, nt); break;
-      case LBM_D3Q19: pf = pick_pair_kernel_d3q19(a->storage, a->collision, a->streaming, nt); break;
-      case LBM_D3Q27: pf = pick_pair_kernel_d3q27(a->storage, a->collision, a->streaming, nt); break;
-    }
-    if (pf) {
-      fn = pf;
-      row_threads = a->n[2] / 2;
-    }
-  }
-  // one block per row with aligned loads and lane shifts along i2
-  bool row_kernel = false;
-  if (fn && !generic && !mask && a->streaming == LBM_PRE_STREAMING && a->n[2] % 64 == 0 && a->n[2] <= 1024 &&
-      env_int("LETTUCE_AMD_ROW_KERNEL", 0) != 0 && row_threads == a->n[2]) {
-    KernelFn rf = nullptr;
-    switch (a->stencil) {
-      case LBM_D2Q9: rf = pick_row_kernel_d2q9(a->storage, a->collision); break;
-      case LBM_D3Q19: rf = pick_row_kernel_d3q19(a->storage, a->collision); break;
-      case LBM_D3Q27: rf = pick_row_kernel_d3q27(a->storage, a->collision); break;
-    }
-    if (rf) {
-      fn = rf;
-      row_kernel = true;
-    }
-  }
-  const int bs = row_kernel ? (int)a->n[2] : block_size_for(row_threads);
-  p.nb2 = (int)((row_threads + bs - 1) / bs);
+  const int bs = block_size_for(a->n[2]);
+  p.nb2 = (int)((a->n[2] + bs - 1) / bs);
   const int64_t nb = (int64_t)p.nb2 * a->n[1] * a->n[0];
   if (nb > 0x7fffffff) return fail(LBM_E_ARG, "too many blocks");
   p.nblocks = (int)nb;
-  p.order = block_order(p, nb);
   const bool learned = a->collision == LBM_COLL_LEARNED;
   const bool guo = a->collision == LBM_COLL_BGK_GUO;
   p.tau_inv = (a->collision == LBM_COLL_BGK || learned || guo) ? 1.0 / a->tau : 0.0;
@@ -344,6 +316,77 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   grid = dim3((unsigned)nb);
   block = dim3(bs);
   return LBM_OK;
+}
+
+// `count` single-step launches of fn, alternating the buffers
+static void launch_singles(KernelFn fn, dim3 grid, dim3 block, KParams p, const void* const (&bufs)[2],
+                           int64_t& launch, int64_t count, hipStream_t s) {
+  for (int64_t k = 0; k < count; ++k, ++launch) {
+    p.fin = bufs[launch & 1];
+    p.fout = (void*)bufs[(launch + 1) & 1];
+    hipLaunchKernelGGL(fn, grid, block, 0, s, p);
+  }
+}
+
+// `nsteps` pull steps of a (a->streaming == LBM_PRE_STREAMING) from
+// bufs[launch & 1]: pairs fused by the two-step kernel where it applies,
+// the rest as single steps.  The number of launches has the parity of
+// nsteps, so the result sits where single steps would leave it.
+static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn, dim3 grid,
+                           dim3 block, const void* const (&bufs)[2], int64_t& launch,
+                           int64_t nsteps, hipStream_t s) {
+  TbFn tb = two_step_kernel(a);
+  const int64_t pairs = two_step_pairs(tb, nsteps);
+  if (pairs > 0) {
+    TbParams tp;
+    tp.k = p;
+    tp.nzt = (int)(a->n[2] / 64);
+    tp.nyt = (int)(a->n[1] / TB_Y);
+    // x chunk: 64 planes, or 128 where 64-plane tiles would make between one
+    // and two rounds of the blocks the device holds at once, so that they
+    // make one.  Measured on MI355X: D3Q19 256^3 (512 tiles, 512 resident
+    // blocks) and 1024^3 (32768 tiles) are fastest at 64 planes (larger
+    // chunks: -4 % to -17 %); D3Q27 KBC FP32 256^3, one block per CU, runs
+    // 30.7k MLUPS at 64 planes and 31.3k at 128.
+    tp.cx = knob(K_TB2_PLANES);
+    if (tp.cx < 1) {
+      tp.cx = 64;
+      int per_cu = 0, dev = 0, cus = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)tb, TB_THREADS, 0) ==
+              hipSuccess &&
+          per_cu > 0) {
+        const int64_t resident = (int64_t)per_cu * cus;
+        const int64_t tiles64 = (int64_t)tp.nzt * tp.nyt * ((a->n[0] + 63) / 64);
+        if (tiles64 > resident && tiles64 <= 2 * resident) tp.cx = 128;
+      }
+    }
+    const int64_t nxt = (a->n[0] + tp.cx - 1) / tp.cx;
+    const int64_t ntiles = (int64_t)tp.nzt * tp.nyt * nxt;
+    tp.ntiles = (int)ntiles;
+    // block b runs on XCD b % 8 and takes tile (b % 8) * per_xcd + b / 8
+    tp.per_xcd = (int)((ntiles + 7) / 8);
+    const int64_t nb = 8 * (int64_t)tp.per_xcd;
+    for (int64_t k = 0; k < pairs; ++k, ++launch) {
+      tp.k.fin = bufs[launch & 1];
+      tp.k.fout = (void*)bufs[(launch + 1) & 1];
+      hipLaunchKernelGGL(tb, dim3((unsigned)nb), dim3(TB_THREADS), 0, s, tp);
+    }
+  }
+  launch_singles(fn, grid, block, p, bufs, launch, nsteps - 2 * pairs, s);
+}
+
+// Whether lbm_run splits a post-streaming run (lettuce's default order,
+// collide then stream) into collide-only, pull and double-streaming launches:
+// for n >= 2 steps without the learned collision, unless the knob
+// post_split is 0.  The two orders are one composition of the same
+// operators, (S C)^n = (S C S) (C S)^(n-2) C, so every cell takes the same
+// values; the pull kernels run faster than the push kernel and can fuse
+// steps in pairs.
+static bool post_split(const lbm_step_args* a, int64_t nsteps) {
+  return a->streaming == LBM_POST_STREAMING && a->collision != LBM_COLL_LEARNED && nsteps >= 2 &&
+         knob(K_POST_SPLIT) != 0;
 }
 
 extern "C" {
@@ -361,6 +404,16 @@ int64_t lbm_struct_size(int which) {
   return -1;
 }
 
+int lbm_set_tuning(const char* name, int32_t value) {
+  if (!name) return fail(LBM_E_ARG, "NULL tuning name");
+  for (Knob& k : g_knobs)
+    if (!strcmp(k.name, name)) {
+      k.value = value < 0 ? k.start : value;
+      return LBM_OK;
+    }
+  return fail(LBM_E_ARG, "unknown tuning knob '%s'", name);
+}
+
 int lbm_step(const lbm_step_args* a, void* stream) {
   return lbm_run(a, 1, stream);
 }
@@ -373,9 +426,18 @@ int lbm_run_plan(const lbm_step_args* a, int64_t nsteps, int64_t* plan) {
   dim3 grid, block;
   int rc = prepare_step(a, p, fn, grid, block);
   if (rc) return rc;
-  const int64_t pairs = a->collision == LBM_COLL_LEARNED ? 0 : two_step_pairs(two_step_kernel(a), nsteps);
-  plan[0] = pairs;
-  plan[1] = nsteps - 2 * pairs;
+  if (a->collision == LBM_COLL_LEARNED) {
+    plan[0] = 0;
+    plan[1] = nsteps;
+  } else if (post_split(a, nsteps)) {
+    lbm_step_args k = *a;
+    k.streaming = LBM_PRE_STREAMING;
+    plan[0] = two_step_pairs(two_step_kernel(&k), nsteps - 2);
+    plan[1] = nsteps - 2 * plan[0];
+  } else {
+    plan[0] = two_step_pairs(two_step_kernel(a), nsteps);
+    plan[1] = nsteps - 2 * plan[0];
+  }
   return LBM_OK;
 }
 
@@ -388,58 +450,36 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   if (rc) return rc;
   if (nsteps == 0) return LBM_OK;
   g_err[0] = 0;
+  hipStream_t s = (hipStream_t)stream;
   if (a->collision == LBM_COLL_LEARNED) {
     rc = launch_learned(p, a->learned, a->stencil, a->storage, a->cell_class != nullptr, a->streaming,
-                        wide_offsets(a->storage, a->n, a->stride), (hipStream_t)stream, nsteps,
-                        a->f_in, a->f_out);
+                        wide_offsets(a->storage, a->n, a->stride), s, nsteps, a->f_in, a->f_out);
     if (rc) return fail(rc, "no learned kernel for storage %d", a->storage);
     return check_launch("lbm_run (learned)");
   }
   const void* bufs[2] = {a->f_in, a->f_out};
   int64_t launch = 0;  // launches so far; launch k reads bufs[k & 1]
-  TbFn tb = two_step_kernel(a);
-  const int64_t pairs = two_step_pairs(tb, nsteps);
-  if (pairs > 0) {
-    TbParams tp;
-    tp.k = p;
-    tp.nzt = (int)(a->n[2] / 64);
-    tp.nyt = (int)(a->n[1] / TB_Y);
-    // x chunk: 64 planes, or 128 where 64-plane tiles would make between one
-    // and two rounds of the blocks the device holds at once, so that they
-    // make one.  Measured on MI355X: D3Q19 256^3 (512 tiles, 512 resident
-    // blocks) and 1024^3 (32768 tiles) are fastest at 64 planes (larger
-    // chunks: -4 % to -17 %); D3Q27 KBC FP32 256^3, one block per CU, runs
-    // 30.7k MLUPS at 64 planes and 31.3k at 128.
-    tp.cx = env_int("LETTUCE_AMD_TB2_PLANES", 0);
-    if (tp.cx < 1) {
-      tp.cx = 64;
-      int per_cu = 0, dev = 0, cus = 0;
-      if (hipGetDevice(&dev) == hipSuccess &&
-          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)tb, TB_THREADS, 0) ==
-              hipSuccess &&
-          per_cu > 0) {
-        const int64_t resident = (int64_t)per_cu * cus;
-        const int64_t tiles64 = (int64_t)tp.nzt * tp.nyt * ((a->n[0] + 63) / 64);
-        if (tiles64 > resident && tiles64 <= 2 * resident) tp.cx = 128;
-      }
-    }
-    const int64_t nxt = (a->n[0] + tp.cx - 1) / tp.cx;
-    const int64_t ntiles = (int64_t)tp.nzt * tp.nyt * nxt;
-    tp.ntiles = (int)ntiles;
-    tp.per_xcd = env_int("LETTUCE_AMD_TB2_XCD", 1) ? (int)((ntiles + 7) / 8) : 0;
-    const int64_t nb = tp.per_xcd ? 8 * (int64_t)tp.per_xcd : ntiles;
-    for (int64_t k = 0; k < pairs; ++k, ++launch) {
-      tp.k.fin = bufs[launch & 1];
-      tp.k.fout = (void*)bufs[(launch + 1) & 1];
-      hipLaunchKernelGGL(tb, dim3((unsigned)nb), dim3(TB_THREADS), 0, (hipStream_t)stream, tp);
-    }
+  if (post_split(a, nsteps)) {
+    // (S C)^n = (S C S) (C S)^(n-2) C: collide only, n - 2 pull steps (two
+    // fused per launch where the two-step kernel applies), then stream,
+    // collide and stream.  n launches as with the push kernel, or fewer.
+    lbm_step_args c = *a, k = *a, d = *a;
+    c.streaming = LBM_NO_STREAMING;
+    k.streaming = LBM_PRE_STREAMING;
+    d.streaming = LBM_DOUBLE_STREAMING;
+    KParams pc, pk, pd;
+    KernelFn fc, fk, fd;
+    dim3 gc, gk, gd, bc, bk, bd;
+    if ((rc = prepare_step(&c, pc, fc, gc, bc)) || (rc = prepare_step(&k, pk, fk, gk, bk)) ||
+        (rc = prepare_step(&d, pd, fd, gd, bd)))
+      return rc;
+    launch_singles(fc, gc, bc, pc, bufs, launch, 1, s);
+    run_pull_steps(&k, pk, fk, gk, bk, bufs, launch, nsteps - 2, s);
+    launch_singles(fd, gd, bd, pd, bufs, launch, 1, s);
+    return check_launch("lbm_run (post-streaming)");
   }
-  for (int64_t k = 2 * pairs; k < nsteps; ++k, ++launch) {
-    p.fin = bufs[launch & 1];
-    p.fout = (void*)bufs[(launch + 1) & 1];
-    hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, p);
-  }
+  if (a->streaming == LBM_PRE_STREAMING) run_pull_steps(a, p, fn, grid, block, bufs, launch, nsteps, s);
+  else launch_singles(fn, grid, block, p, bufs, launch, nsteps, s);
   return check_launch("lbm_run");
 }
 
@@ -499,7 +539,7 @@ int lbm_slab_step(const lbm_step_args* a, int32_t part, const void* recv_lo, con
   KParams p;
   KernelFn fn;
   dim3 grid, block;
-  int rc = prepare_step(a, p, fn, grid, block, true);
+  int rc = prepare_step(a, p, fn, grid, block);
   if (rc) return rc;
   rc = check_slab_extent(a->stencil, a->n);
   if (rc) return rc;
@@ -526,7 +566,6 @@ int lbm_slab_step(const lbm_step_args* a, int32_t part, const void* recv_lo, con
     pi.x0 = 1;
     const int64_t nb = (int64_t)p.nb2 * a->n[1] * (n0 - 2);
     pi.nblocks = (int)nb;
-    pi.order = block_order(pi, nb);
     hipLaunchKernelGGL(fn, dim3((unsigned)nb), block, 0, (hipStream_t)stream, pi);
   }
   if (faces) {

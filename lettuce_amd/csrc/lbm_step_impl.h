@@ -31,7 +31,6 @@ struct KParams {
   long long ncells;   // n0*n1*n2 (bc_field plane stride)
   int nb2;            // blocks along i2
   int nblocks;        // total blocks
-  int order;          // block order, one of BlockOrder
   double tau_inv, trt_p, trt_m, beta, beta_inv;
   double rho0;
   double accel[3], guo;  // BGK_GUO
@@ -39,27 +38,11 @@ struct KParams {
   int nclasses = 0;       // entries of ct (0 without boundaries)
 };
 
-// Which row (i0, i1) and which run b2 of the row a block takes.  The
-// hardware hands block b to XCD b % 8, and each XCD has its own L2:
-//  LINEAR     rows in storage order (i1 fastest);
-//  XCD_SLAB   XCD k sweeps the k-th eighth of the blocks in storage order,
-//             so its i1 +- 1 neighbour rows stay in its L2;
-//  XCD_CHUNK  XCD k owns rows i1 in the k-th eighth of [0, n1) and walks
-//             them with i0 slowest, so both the i1 +- 1 rows and the
-//             i0 +- 1 planes of its chunk were read moments before.
-enum BlockOrder { ORDER_LINEAR = 0, ORDER_XCD_SLAB = 1, ORDER_XCD_CHUNK = 2 };
-
+// Which row (i0, i1) and which run b2 of the row a block takes: rows in
+// storage order, i1 fastest.  (XCD-aware orders, in which each of the 8
+// XCDs sweeps its own contiguous share of the rows, measured slower on
+// MI355X for D3Q19 256^3 FP32: 5.57 and 4.76 TB/s against 5.69 TB/s.)
 __device__ __forceinline__ void block_tile(const KParams& p, int b, int& i0, int& i1, int& b2) {
-  if (p.order == ORDER_XCD_CHUNK) {
-    const int k = b & 7, j = b >> 3;
-    const int h = p.n1 >> 3;  // rows per XCD chunk
-    b2 = j % p.nb2;
-    const int t = j / p.nb2;
-    i1 = k * h + t % h;
-    i0 = p.x0 + t / h;
-    return;
-  }
-  if (p.order == ORDER_XCD_SLAB) b = (b & 7) * (p.nblocks >> 3) + (b >> 3);
   b2 = b % p.nb2;
   const int r = b / p.nb2;
   i1 = r % p.n1;
@@ -492,77 +475,6 @@ __global__ void __launch_bounds__(256) step_kernel(const KParams p) {
 }
 
 
-// Unmasked pull (or no-streaming) step with two neighbouring cells per
-// thread along i2: every population moves as one 2-element load and one
-// 2-element store (global_load_dwordx2 for FP32).  A pull along i2 reads the
-// pair shifted by one element (4-byte aligned for FP32, which gfx950 loads
-// directly); only the pair that wraps around the periodic row end reads two
-// single elements.  NT selects streaming (non-temporal) stores.
-template <class S, class ST, int COLL, bool PULL, bool NT>
-__global__ void __launch_bounds__(256) pair_kernel(const KParams p) {
-  using C = typename ST::C;
-  using T = typename ST::T;
-  typedef T T2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
-  int i0, i1, b2;
-  block_tile(p, blockIdx.x, i0, i1, b2);
-  const int z0 = 2 * (b2 * (int)blockDim.x + (int)threadIdx.x);
-  if (z0 >= p.n2) return;
-  long long a0[3];
-  int a1[3];
-  axis_offsets(i0, p.n0, p.s0, a0);
-  axis_offsets(i1, p.n1, p.s1, a1);
-  const bool lo_edge = z0 == 0, hi_edge = z0 + 2 == p.n2;
-  const T* fin = (const T*)p.fin;
-  C f0[S::Q], f1[S::Q];
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    const C sh = shift_of<ST>(S::w[q], p);
-    const int ez = PULL ? go<S>(q, 2) : 0;
-    const T* row = fin + q * p.sq + (PULL ? a0[1 - go<S>(q, 0)] + a1[1 - go<S>(q, 1)]
-                                          : a0[1] + a1[1]);
-    T x0, x1;
-    if (ez == 0) {
-      const T2 v = *(const T2*)(row + z0);
-      x0 = v.x;
-      x1 = v.y;
-    } else if (ez > 0) {  // cells z0, z0+1 pull from z0-1, z0
-      if (!lo_edge) {
-        const T2 v = *(const T2*)(row + z0 - 1);
-        x0 = v.x;
-        x1 = v.y;
-      } else {
-        x0 = row[p.n2 - 1];
-        x1 = row[z0];
-      }
-    } else {  // cells z0, z0+1 pull from z0+1, z0+2
-      if (!hi_edge) {
-        const T2 v = *(const T2*)(row + z0 + 1);
-        x0 = v.x;
-        x1 = v.y;
-      } else {
-        x0 = row[z0 + 1];
-        x1 = row[0];
-      }
-    }
-    f0[q] = ST::dec(x0, sh);
-    f1[q] = ST::dec(x1, sh);
-  }
-  const CollParams<C> cp = coll_params<C>(p);
-  collide<S, C, COLL>(f0, cp);
-  collide<S, C, COLL>(f1, cp);
-  T* fout = (T*)p.fout + a0[1] + a1[1] + z0;
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    const C sh = shift_of<ST>(S::w[q], p);
-    T2 v;
-    v.x = ST::enc(f0[q], sh);
-    v.y = ST::enc(f1[q], sh);
-    T2* dst = (T2*)(fout + q * p.sq);
-    if (NT) __builtin_nontemporal_store(v, dst);
-    else *dst = v;
-  }
-}
-
 // Shift a value across the lanes of the wavefront with DPP: DIR = +1 gives
 // lane i the value of lane i - 1 (wave_shr:1), DIR = -1 the value of lane
 // i + 1 (wave_shl:1).  The lane at the wave's edge keeps its own value.
@@ -585,119 +497,10 @@ __device__ __forceinline__ T wave_shift(T v) {
   }
 }
 
-// Unmasked pull step with one block per row along i2 (blockDim.x == n2, a
-// multiple of 64): every population is loaded aligned at the thread's own
-// i2; the +-1 shift along i2 is a lane shift inside the wavefront, and the
-// element that crosses a wavefront (or the periodic row end) goes through
-// two LDS slots per wave and population.  Same arithmetic as step_kernel.
-template <class S, class ST, int COLL>
-__global__ void __launch_bounds__(1024) row_kernel(const KParams p) {
-  using C = typename ST::C;
-  using T = typename ST::T;
-  __shared__ T edge_last[S::Q][16];   // last element of each wave
-  __shared__ T edge_first[S::Q][16];  // first element of each wave
-  const int row = blockIdx.x;
-  const int i1 = row % p.n1, i0 = row / p.n1;
-  const int z = threadIdx.x, lane = z & 63, wid = z >> 6;
-  const int nw = blockDim.x >> 6;
-  long long a0[3];
-  int a1[3];
-  axis_offsets(i0, p.n0, p.s0, a0);
-  axis_offsets(i1, p.n1, p.s1, a1);
-  const T* fin = (const T*)p.fin;
-  T raw[S::Q];
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    const T v = (fin + q * p.sq + a0[1 - go<S>(q, 0)])[a1[1 - go<S>(q, 1)] + z];
-    const int ez = go<S>(q, 2);
-    if (ez == 0) {
-      raw[q] = v;
-    } else if (ez > 0) {  // needs z - 1
-      raw[q] = wave_shift<+1>(v);
-      if (lane == 63) edge_last[q][wid] = v;
-    } else {  // needs z + 1
-      raw[q] = wave_shift<-1>(v);
-      if (lane == 0) edge_first[q][wid] = v;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) {
-    const int ez = go<S>(q, 2);
-    if (ez > 0 && lane == 0) raw[q] = edge_last[q][wid == 0 ? nw - 1 : wid - 1];
-    if (ez < 0 && lane == 63) raw[q] = edge_first[q][wid == nw - 1 ? 0 : wid + 1];
-  }
-  C f[S::Q];
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q) f[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
-  collide<S, C, COLL>(f, coll_params<C>(p));
-  T* fout = (T*)p.fout;
-#pragma unroll
-  for (int q = 0; q < S::Q; ++q)
-    ST::st(fout + q * p.sq + a0[1], a1[1] + z, f[q], shift_of<ST>(S::w[q], p));
-}
-
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 using KernelFn = void (*)(const KParams);
-
-template <class S, class ST>
-KernelFn pick_row_coll(int coll) {
-  switch (coll) {
-    case LBM_COLL_NONE: return row_kernel<S, ST, LBM_COLL_NONE>;
-    case LBM_COLL_BGK: return row_kernel<S, ST, LBM_COLL_BGK>;
-    case LBM_COLL_TRT: return row_kernel<S, ST, LBM_COLL_TRT>;
-    case LBM_COLL_KBC:
-      if constexpr (S::ID == LBM_D3Q19) return nullptr;
-      else return row_kernel<S, ST, LBM_COLL_KBC>;
-  }
-  return nullptr;
-}
-
-// row kernels exist for the unmasked pull path of every collision but the
-// learned one
-template <class S>
-KernelFn pick_row_kernel(int storage, int coll) {
-  switch (storage) {
-    case LBM_F32: return pick_row_coll<S, StF32>(coll);
-    case LBM_F64: return pick_row_coll<S, StF64>(coll);
-    case LBM_F16_SHIFTED: return pick_row_coll<S, StF16>(coll);
-    case LBM_BF16_SHIFTED: return pick_row_coll<S, StBF16>(coll);
-  }
-  return nullptr;
-}
-
-template <class S, class ST, int COLL>
-KernelFn pick_pair_st(bool pull, bool nt) {
-  if (pull) return nt ? pair_kernel<S, ST, COLL, true, true> : pair_kernel<S, ST, COLL, true, false>;
-  return nt ? pair_kernel<S, ST, COLL, false, true> : pair_kernel<S, ST, COLL, false, false>;
-}
-
-template <class S, class ST>
-KernelFn pick_pair_coll(int coll, bool pull, bool nt) {
-  switch (coll) {
-    case LBM_COLL_NONE: return pick_pair_st<S, ST, LBM_COLL_NONE>(pull, nt);
-    case LBM_COLL_BGK: return pick_pair_st<S, ST, LBM_COLL_BGK>(pull, nt);
-    case LBM_COLL_TRT: return pick_pair_st<S, ST, LBM_COLL_TRT>(pull, nt);
-  }
-  return nullptr;
-}
-
-// pair kernels exist for the unmasked pull / no-streaming paths of BGK, TRT
-// and no collision
-template <class S>
-KernelFn pick_pair_kernel(int storage, int coll, int mode, bool nt) {
-  if (mode != LBM_PRE_STREAMING && mode != LBM_NO_STREAMING) return nullptr;
-  const bool pull = mode == LBM_PRE_STREAMING;
-  switch (storage) {
-    case LBM_F32: return pick_pair_coll<S, StF32>(coll, pull, nt);
-    case LBM_F64: return pick_pair_coll<S, StF64>(coll, pull, nt);
-    case LBM_F16_SHIFTED: return pick_pair_coll<S, StF16>(coll, pull, nt);
-    case LBM_BF16_SHIFTED: return pick_pair_coll<S, StBF16>(coll, pull, nt);
-  }
-  return nullptr;
-}
 
 template <class S, class ST, int COLL, bool MASK>
 KernelFn pick_mode(int mode) {

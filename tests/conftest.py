@@ -26,3 +26,19 @@ def gpu_device():
     if not torch.cuda.is_available():
         pytest.fail("GPU tests need a ROCm device")
     return torch.device("cuda:0")
+
+
+@pytest.fixture
+def knobs():
+    """Setter for the library's launch-tuning knobs (lbm_set_tuning): call
+    knobs(two_step=0, ...); every knob set is restored at teardown."""
+    from lettuce_amd import _native as N
+    touched = set()
+
+    def set_knobs(**kw):
+        for name, value in kw.items():
+            N.check(N.lib().lbm_set_tuning(name.encode(), int(value)), "lbm_set_tuning")
+            touched.add(name)
+    yield set_knobs
+    for name in touched:
+        N.lib().lbm_set_tuning(name.encode(), -1)

@@ -29,14 +29,14 @@ def _sums(flow):
     return _ops.reduce_f64(flow.stencil, flow._state, flow.rho0_shift).cpu().numpy()
 
 
-def test_headline_two_step_matches_single_steps_at_256(monkeypatch):
+def test_headline_two_step_matches_single_steps_at_256(knobs):
     """D3Q19 BGK TGV 256^3 FP32 (the bench workload), 20 steps: the default
     path (10 two-step launches) against 20 single-step launches, and mass
     and momentum conserved"""
     S = lt.D3Q19()
     runs = {}
     for mode in ("1", "0"):
-        monkeypatch.setenv("LETTUCE_AMD_TB2", mode)
+        knobs(two_step=mode)
         flow = _tgv(S, 256)
         sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
                             lt.StreamingStrategy.PRE_STREAMING)

@@ -450,18 +450,16 @@ def test_run_batches_equal_single_steps():
 
 @pytest.mark.parametrize("name,res", [("D3Q19", [8, 6, 10]), ("D2Q9", [12, 2]),
                                       ("D3Q27", [4, 4, 6]), ("D2Q9", [5, 7]),
-                                      ("D3Q19", [4, 6, 128]), ("D2Q9", [12, 64]),
-                                      ("D3Q27", [3, 4, 192])])
+                                      ("D3Q19", [4, 16, 64]), ("D3Q27", [6, 16, 64])])
 @pytest.mark.parametrize("storage,dtype", [("f32", F32), ("f64", F64),
                                            ("bf16_shifted", F32)])
-@pytest.mark.parametrize("variant", ["LETTUCE_AMD_CELLS_PER_THREAD", "LETTUCE_AMD_ROW_KERNEL"])
-def test_kernel_variants_match_single_cell_kernel(name, res, storage, dtype, variant,
-                                                  monkeypatch):
-    """The two-cells-per-thread kernel and the row kernel (aligned loads and
-    lane shifts, rows of 64..1024) give the same populations as the
-    one-cell-per-thread kernel bit for bit (same per-cell arithmetic; only
-    the memory access differs), including the periodic wrap at the row ends
-    and rows the variants do not apply to."""
+def test_split_post_streaming_run_matches_push_steps(name, res, storage, dtype, knobs):
+    """lettuce's default order (collide, then stream) run as one collide-only
+    launch, pull steps (two-step kernel where it applies: the 16 x 64 grids)
+    and one double-streaming launch gives the populations of the same number
+    of push steps: bit for bit without collision, else to the rounding of
+    the collision arithmetic, which the compiler may contract differently in
+    different kernels"""
     s, os_ = _stencils()[name]
     g = torch.Generator().manual_seed(9)
     from tests.test_oracle import _Grid
@@ -469,19 +467,45 @@ def test_kernel_variants_match_single_cell_kernel(name, res, storage, dtype, var
     u = 0.05 * (torch.rand([s.d] + res, generator=g, dtype=F64) - 0.5)
     f0 = oflow.equilibrium(oflow, rho=torch.ones([1] + res, dtype=F64), u=u)
     f0 = (f0 * (1 + 0.02 * torch.rand(f0.shape, generator=g, dtype=F64))).numpy()
-    outs = []
-    monkeypatch.setenv("LETTUCE_AMD_CELLS_PER_THREAD", "1")
-    monkeypatch.setenv("LETTUCE_AMD_ROW_KERNEL", "0")
-    for on in ("0", "2" if variant == "LETTUCE_AMD_CELLS_PER_THREAD" else "1"):
-        if on != "0":
-            monkeypatch.setenv(variant, on)
-        for strategy in ("PRE_STREAMING", "NO_STREAMING"):
+    ulp = {"f32": 2e-7, "f64": 1e-15, "bf16_shifted": 1e-4}[storage]
+    for make in (lambda: lt.NoCollision(),
+                 lambda: lt.TRTCollision(0.7, 0.9) if name == "D3Q19" else lt.KBCCollision()):
+        outs = []
+        for split in (0, 1):
+            knobs(post_split=split)
             flow = ArrayFlow(ctx(dtype, storage), f0, s)
-            coll = lt.TRTCollision(0.7, 0.9) if name == "D3Q19" else lt.KBCCollision()
-            if variant == "LETTUCE_AMD_CELLS_PER_THREAD":
-                coll = lt.BGKCollision(0.7)   # pair kernels: BGK, TRT, none
-            outs.append(native_run(flow, coll, 5, getattr(lt.StreamingStrategy, strategy)))
-    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+            sim = lt.Simulation(flow, make(), [], lt.StreamingStrategy.POST_STREAMING)
+            pairs, singles = sim._collide_and_stream.plan(9)
+            assert 2 * pairs + singles == 9 and (pairs == 0 or split == 1)
+            sim(9)
+            outs.append(flow._state.float().cpu().numpy() if storage == "bf16_shifted"
+                        else flow._state.cpu().numpy())
+        if isinstance(make(), lt.NoCollision):
+            assert np.array_equal(outs[0], outs[1])
+        else:
+            # KBC's gamma is a ratio of small sums: a rounding difference moves
+            # it further than in BGK / TRT
+            tol = 9 * ulp * (50 if name != "D3Q19" else 1)
+            assert np.abs(outs[0] - outs[1]).max() <= tol
+
+
+def test_split_post_streaming_channel_matches_push_steps(knobs):
+    """A channel with a cylinder (halfway bounce-back, equilibrium inlet and
+    outlet), 12 post-streaming steps: split run against push steps"""
+    outs = []
+    for split in (0, 1):
+        knobs(post_split=split)
+        flow = lt.Obstacle(lt.Context("cuda:0", dtype=F64), [96, 40], 100, 0.1,
+                           domain_length_x=96 / 16, stencil=lt.D2Q9(), outlet="equilibrium",
+                           solid="halfway", walls=True)
+        yy, xx = np.meshgrid(np.arange(40), np.arange(96))
+        flow.mask = (xx - 24) ** 2 + (yy - 21) ** 2 < 6.0 ** 2
+        flow.initialize()
+        sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                            lt.StreamingStrategy.POST_STREAMING)
+        sim(12)
+        outs.append(flow.f.cpu().numpy())
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-14
 
 
 @pytest.mark.parametrize("name,storage,dtype", [("D3Q19", "f32", F32), ("D2Q9", "f64", F64),

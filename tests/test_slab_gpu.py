@@ -31,9 +31,9 @@ def _sim(f0, storage, layout, coll):
     return lt.Simulation(flow, COLLS[coll](), [], lt.StreamingStrategy.PRE_STREAMING)
 
 
-def _periodic(f0, storage, layout, coll, steps, monkeypatch):
+def _periodic(f0, storage, layout, coll, steps, knobs):
     # single fused steps (the two-step kernel rounds its collisions apart)
-    monkeypatch.setenv("LETTUCE_AMD_TB2", "0")
+    knobs(two_step="0")
     sim = _sim(f0, storage, layout, coll)
     sim(steps)
     torch.cuda.synchronize()
@@ -88,26 +88,26 @@ def _compare(flow_ref, flow, out, storage, coll):
 @pytest.mark.parametrize("storage,layout", [("f32", "soa"), ("f32", "rows"),
                                             ("f16_shifted", "soa"), ("bf16_shifted", "rows")])
 @pytest.mark.parametrize("coll", ["none", "bgk", "trt", "kbc"])
-def test_slabs_step_as_the_periodic_lattice(stencil, storage, layout, coll, monkeypatch):
+def test_slabs_step_as_the_periodic_lattice(stencil, storage, layout, coll, knobs):
     if stencil == "D3Q19" and coll == "kbc":
         pytest.skip("KBC is defined for D2Q9 and D3Q27")
     from oracle import lettuce_torch as O
     res = [48, 16, 24]
     f0 = _f0(res, seed=7, stencil=getattr(O, stencil))
     steps = 6
-    flow_ref = _periodic(f0, storage, layout, coll, steps, monkeypatch)
+    flow_ref = _periodic(f0, storage, layout, coll, steps, knobs)
     for split in (False, True):
         flow, out = _run_slabs(f0, storage, layout, coll, steps, [20, 1, 2, 25], split)
         _compare(flow_ref, flow, out, storage, coll)
 
 
 @pytest.mark.parametrize("stencil,coll", [("D3Q19", "none"), ("D3Q19", "bgk"), ("D3Q27", "kbc")])
-def test_one_slab_wrapped_onto_itself(stencil, coll, monkeypatch):
+def test_one_slab_wrapped_onto_itself(stencil, coll, knobs):
     from oracle import lettuce_torch as O
     res = [12, 8, 64]
     f0 = _f0(res, seed=3, stencil=getattr(O, stencil))
     steps = 9
-    flow_ref = _periodic(f0, "f32", "soa", coll, steps, monkeypatch)
+    flow_ref = _periodic(f0, "f32", "soa", coll, steps, knobs)
     sim = _sim(f0, "f32", "soa", coll)
     slab = lt.SlabStep(sim, sim.flow._state.clone())
     slab.pack()

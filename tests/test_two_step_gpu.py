@@ -72,16 +72,16 @@ ULPS = {"f32": 4 * 0.4 * 2.0 ** -23, "f16_shifted": 4 * 0.05 * 2.0 ** -10,
 @pytest.mark.parametrize("coll", ["bgk", "trt", "none"])
 @pytest.mark.parametrize("barriers", ["1", "2"])
 def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, coll, barriers,
-                                                     monkeypatch):
-    monkeypatch.setenv("LETTUCE_AMD_TB2_BARRIERS", barriers)
+                                                     knobs):
+    knobs(two_step_barriers=barriers)
     f0 = _f0(res)
     make = {"bgk": lambda: lt.BGKCollision(0.62), "trt": lambda: lt.TRTCollision(0.7, 0.9),
             "none": lambda: lt.NoCollision()}[coll]
     for steps in (2, 5, 6, 8):
-        monkeypatch.setenv("LETTUCE_AMD_TB2", "0")
+        knobs(two_step="0")
         ref_bits, ref = _run(f0, storage, layout, make(), steps, pairs=0)
-        monkeypatch.setenv("LETTUCE_AMD_TB2", "2")
-        monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", str(planes))
+        knobs(two_step="2")
+        knobs(two_step_planes=str(planes))
         pairs = steps // 2 - (steps // 2) % 2
         got_bits, got = _run(f0, storage, layout, make(), steps, pairs=pairs)
         if coll == "none":
@@ -92,13 +92,13 @@ def test_two_step_kernel_matches_two_single_steps(res, planes, storage, layout, 
 
 
 @pytest.mark.parametrize("layout", ["soa", "rows"])
-def test_two_step_kernel_matches_the_oracle(layout, monkeypatch):
+def test_two_step_kernel_matches_the_oracle(layout, knobs):
     """FP32 BGK with the default two-step path, 40 steps, against the oracle's
     BGK (lettuce's torch step) to 1e-5"""
     res = [8, 16, 64]
     f0 = _f0(res, seed=2)
-    monkeypatch.setenv("LETTUCE_AMD_TB2", "1")
-    monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", "3")
+    knobs(two_step="1")
+    knobs(two_step_planes="3")
     _, got = _run(f0, "f32", layout, lt.BGKCollision(0.58), 40, pairs=20)
     oflow = _Grid(O.D3Q19(), res, F32, f0.astype(np.float32))
     O.Simulation(oflow, O.BGKCollision(0.58), [], O.StreamingStrategy.PRE_STREAMING)(40)
@@ -106,10 +106,13 @@ def test_two_step_kernel_matches_the_oracle(layout, monkeypatch):
     assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5
 
 
-def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
+def test_two_step_kernel_applies_only_where_it_can(knobs):
     """D3Q19 in every storage format by default, D3Q27 in FP32 only; no
-    two-step launch with push streaming or extents that do not tile"""
-    monkeypatch.delenv("LETTUCE_AMD_TB2", raising=False)
+    two-step launch with the push kernel or extents that do not tile; a
+    post-streaming run of 8 steps splits into one collide-only launch, 6
+    pull steps (2 two-step launches + 2 single steps, an even number of
+    pairs) and one double-streaming launch"""
+    knobs(two_step=-1)
     f0 = _f0([4, 16, 64])
     for storage, pairs in (("f32", 4), ("f16_shifted", 4), ("bf16_shifted", 4)):
         c = lt.Context("cuda:0", dtype=F32, storage=storage)
@@ -119,7 +122,11 @@ def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
     c = lt.Context("cuda:0", dtype=F32)
     sim = lt.Simulation(_Flow(c, f0.astype(np.float32)), lt.BGKCollision(0.6), [],
                         lt.StreamingStrategy.POST_STREAMING)
+    assert sim._collide_and_stream.plan(8) == (2, 4)
+    assert sim._collide_and_stream.plan(1) == (0, 1)
+    knobs(post_split=0)
     assert sim._collide_and_stream.plan(8) == (0, 8)
+    knobs(post_split=-1)
     f1 = _f0([4, 12, 64])
     sim = lt.Simulation(_Flow(c, f1.astype(np.float32)), lt.BGKCollision(0.6), [],
                         lt.StreamingStrategy.PRE_STREAMING)
@@ -136,15 +143,15 @@ def test_two_step_kernel_applies_only_where_it_can(monkeypatch):
 
 @pytest.mark.parametrize("coll", ["kbc", "bgk"])
 @pytest.mark.parametrize("storage,layout", [("f32", "soa"), ("f32", "rows"), ("f16_shifted", "soa")])
-def test_two_step_kernel_d3q27(coll, storage, layout, monkeypatch):
+def test_two_step_kernel_d3q27(coll, storage, layout, knobs):
     """D3Q27 (KBC, BGK): two-step launches against single steps"""
     f0 = _f0([6, 16, 64], stencil=O.D3Q27)
     make = {"kbc": lambda: lt.KBCCollision(), "bgk": lambda: lt.BGKCollision(0.62)}[coll]
     for steps in (4, 9):
-        monkeypatch.setenv("LETTUCE_AMD_TB2", "0")
+        knobs(two_step="0")
         _, ref = _run(f0, storage, layout, make(), steps, pairs=0)
-        monkeypatch.setenv("LETTUCE_AMD_TB2", "2")
-        monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", "4")
+        knobs(two_step="2")
+        knobs(two_step_planes="4")
         pairs = steps // 2 - (steps // 2) % 2
         _, got = _run(f0, storage, layout, make(), steps, pairs=pairs)
         err = np.abs(got - ref).max()
@@ -157,7 +164,7 @@ def test_two_step_kernel_d3q27(coll, storage, layout, monkeypatch):
 @pytest.mark.parametrize("stencil,coll,storage,layout", [
     (O.D3Q19, "none", "f32", "soa"), (O.D3Q19, "bgk", "f32", "rows"),
     (O.D3Q19, "bgk", "bf16_shifted", "rows"), (O.D3Q27, "kbc", "f32", "soa")])
-def test_wide_two_step_kernel_matches_the_narrow_one(stencil, coll, storage, layout, monkeypatch):
+def test_wide_two_step_kernel_matches_the_narrow_one(stencil, coll, storage, layout, knobs):
     """the wide variant (each cell at a 64-bit plane offset plus a 32-bit
     offset inside the plane, which grids of more than 2^31 elements use)
     forced on a small grid, against the narrow variant (one 32-bit offset):
@@ -165,11 +172,11 @@ def test_wide_two_step_kernel_matches_the_narrow_one(stencil, coll, storage, lay
     f0 = _f0([10, 16, 128], stencil=stencil)
     make = {"bgk": lambda: lt.BGKCollision(0.62), "kbc": lambda: lt.KBCCollision(),
             "none": lambda: lt.NoCollision()}[coll]
-    monkeypatch.setenv("LETTUCE_AMD_TB2", "2")
-    monkeypatch.setenv("LETTUCE_AMD_TB2_PLANES", "4")
+    knobs(two_step="2")
+    knobs(two_step_planes="4")
     out = {}
     for wide in ("0", "1"):
-        monkeypatch.setenv("LETTUCE_AMD_WIDE", wide)
+        knobs(wide_offsets=wide)
         out[wide] = _run(f0, storage, layout, make(), 8, pairs=4)
     if coll == "none":
         assert np.array_equal(out["1"][0], out["0"][0])
