@@ -181,6 +181,75 @@ class PoiseuilleFlow2D(ExtFlow):
         return self.context.convert_to_tensor([0.001, 0.0])
 
 
+def turbulence_velocity(resolution, k0, ic_energy, dx, length_pu, randseed, device):
+    """Velocity field (PU) of lettuce's DecayingTurbulence
+    (lettuce/ext/_flows/decayingturbulence.py:67-155), its shell spectrum and
+    shell wavenumbers.  The random phases come from numpy's legacy global
+    RNG seeded with `randseed`, as in lettuce (:90-91); the FFTs and the
+    projection run in FP64 on `device` with torch, which agrees with
+    lettuce's numpy arithmetic to rounding.  Wavenumbers on np.meshgrid's
+    default 'xy' grid, shells k - 1/2 < |k| <= k + 1/2 (:67-76);
+    E(k) ~ k^4 exp(-2 (k/k0)^2) normalised to ic_energy (:78-86); amplitudes
+    sqrt(2/d E / |u_hat|^2) (:102-110); divergence removed with the
+    modified wavenumber sin(k dx)/dx (:118-134); energy rescaled to
+    ic_energy (:136-143); the inverse transform scaled as in :145-153."""
+    dev = torch.device(device)
+    f64 = torch.float64
+    dims = tuple(int(n) for n in resolution)
+    d = len(dims)
+    axes = tuple(range(d))
+    freqs = [torch.fft.fftfreq(n, d=1.0 / n, dtype=f64, device=dev) for n in dims]
+    wavenumber = torch.stack(torch.meshgrid(*freqs, indexing="xy"))
+    norms = torch.linalg.vector_norm(wavenumber, dim=0)
+    ek = norms ** 4 * torch.exp(-2 * (norms / k0) ** 2)
+    ek = ek / ek.sum()
+    ek = ek * ic_energy
+    # shell k holds the modes with k - 1/2 < |k| <= k + 1/2, k < int(max |k|)
+    nshell = int(norms.max())
+    shell = torch.ceil(norms - 0.5).long()
+    inside = shell < nshell
+    spectrum = torch.zeros(nshell, dtype=f64, device=dev).index_add_(
+        0, shell[inside], ek[inside])
+    del shell, inside
+    np.random.seed(randseed)
+    phases = np.random.random(tuple(wavenumber.shape)) * 2 * np.pi
+    re, im = [], []
+    for k in range(d):
+        uh = torch.fft.fftn(torch.from_numpy(phases[k]).to(dev).to(torch.complex128), dim=axes)
+        re.append(uh.real.contiguous())
+        im.append(uh.imag.contiguous())
+    del phases
+    for k in range(d):
+        amp = torch.sqrt(2 / d * ek / (im[k] ** 2 + re[k] ** 2 + 1.e-15))
+        re[k] = amp * re[k]
+        im[k] = amp * im[k]
+        re[k].view(-1)[0] = 0
+        im[k].view(-1)[0] = 0
+    kmod = [torch.sin(wavenumber[k] * dx) / dx for k in range(d)]
+    del wavenumber, norms, ek
+    knorm = torch.linalg.vector_norm(torch.stack(kmod), dim=0) + 1e-16
+    div_re = torch.zeros(re[0].shape, dtype=f64, device=dev)
+    div_im = torch.zeros(re[0].shape, dtype=f64, device=dev)
+    for k in range(d):
+        div_re += kmod[k] * re[k]
+        div_im += kmod[k] * im[k]
+    for k in range(d):
+        re[k] = re[k] - div_re * kmod[k] / knorm ** 2
+        im[k] = im[k] - div_im * kmod[k] / knorm ** 2
+        re[k].view(-1)[0] = 0
+        im[k].view(-1)[0] = 0
+    del div_re, div_im, kmod, knorm
+    e_kin = 0.5 * sum(float(torch.sum(re[k] ** 2 + im[k] ** 2)) for k in range(d))
+    factor = np.sqrt(ic_energy / e_kin)
+    if d == 3:
+        norm = dims[0] * dx ** (1 - d) * np.sqrt(length_pu)
+    else:
+        norm = dims[0] / dx
+    u = torch.stack([(torch.fft.ifftn(torch.complex(re[k] * factor, im[k] * factor), dim=axes)
+                      * norm).real for k in range(d)])
+    return u, spectrum.cpu().numpy(), np.arange(nshell)
+
+
 class DecayingTurbulence(ExtFlow):
     """Decaying turbulence with E(k) ~ k^4 exp(-2 (k/k0)^2), random phases
     from numpy's legacy RNG, a solenoidal projection with the modified
@@ -218,76 +287,14 @@ class DecayingTurbulence(ExtFlow):
             characteristic_length_pu=2 * np.pi,
             characteristic_velocity_pu=None)
 
-    def _wavenumbers(self):
-        dims = tuple(self.resolution)
-        freqs = [np.fft.fftfreq(n, d=1 / n) for n in dims]
-        wavenumber = np.meshgrid(*freqs)
-        norms = np.linalg.norm(wavenumber, axis=0)
-        self.wavenumbers = np.arange(int(np.max(norms)))
-        shells = ((norms[..., None] > self.wavenumbers - 0.5)
-                  & (norms[..., None] <= self.wavenumbers + 0.5))
-        return norms, wavenumber, shells
-
-    def _spectrum(self):
-        norms, wavenumber, shells = self._wavenumbers()
-        ek = norms ** 4 * np.exp(-2 * (norms / self.k0) ** 2)
-        ek /= np.sum(ek)
-        ek *= self.ic_energy
-        self.spectrum = np.sum(ek[..., None] * shells,
-                               axis=tuple(range(self.stencil.d)))
-        return ek, wavenumber
-
-    def _velocity(self, ek, wavenumber):
-        d = self.stencil.d
-        dims = tuple(self.resolution)
-        axes = tuple(range(d))
-        dx = self.units.convert_length_to_pu(1.0)
-        np.random.seed(self.randseed)
-        phases = np.random.random(np.array(wavenumber).shape) * 2 * np.pi + 0j
-        uh = [np.fft.fftn(phases[k], axes=axes) for k in range(d)]
-        re = [c.real for c in uh]
-        im = [c.imag for c in uh]
-        for k in range(d):
-            re[k].ravel()[0] = 0
-            im[k].ravel()[0] = 0
-        amp = [np.sqrt(2 / d * ek / (im[k] ** 2 + re[k] ** 2 + 1.e-15))
-               for k in range(d)]
-        re = [amp[k] * re[k] for k in range(d)]
-        im = [amp[k] * im[k] for k in range(d)]
-        for k in range(d):
-            re[k].ravel()[0] = 0
-            im[k].ravel()[0] = 0
-        # remove divergence with the modified wavenumber sin(k dx)/dx of the
-        # second-order central difference
-        kmod = [np.sin(wavenumber[k] * dx) / dx for k in range(d)]
-        knorm = np.linalg.norm(kmod, axis=0) + 1e-16
-        div_re = np.zeros(dims)
-        div_im = np.zeros(dims)
-        for k in range(d):
-            div_re += kmod[k] * re[k]
-            div_im += kmod[k] * im[k]
-        re = [re[k] - div_re * kmod[k] / knorm ** 2 for k in range(d)]
-        im = [im[k] - div_im * kmod[k] / knorm ** 2 for k in range(d)]
-        for k in range(d):
-            re[k].ravel()[0] = 0
-            im[k].ravel()[0] = 0
-        e_kin = 0.5 * np.sum([np.sum(re[k] ** 2 + im[k] ** 2) for k in range(d)])
-        factor = np.sqrt(self.ic_energy / e_kin)
-        re = [r * factor for r in re]
-        im = [i * factor for i in im]
-        if d == 3:
-            norm = (self.resolution[0] * dx ** (1 - d)
-                    * np.sqrt(self.units.characteristic_length_pu))
-        else:
-            norm = self.resolution[0] / dx
-        return np.asarray([(np.fft.ifftn(re[k] + im[k] * 1.0j, axes=axes)
-                            * norm).real for k in range(d)])
-
     def initial_pu(self):
-        ek, wavenumber = self._spectrum()
-        u = self._velocity(ek, wavenumber)
-        self.units.characteristic_velocity_pu = np.linalg.norm(u, axis=0).max()
-        p = np.zeros(tuple(self.resolution))[None, ...]
+        dx = self.units.convert_length_to_pu(1.0)
+        u, self.spectrum, self.wavenumbers = turbulence_velocity(
+            self.resolution, self.k0, self.ic_energy, dx,
+            self.units.characteristic_length_pu, self.randseed, self.context.device)
+        self.units.characteristic_velocity_pu = float(
+            torch.linalg.vector_norm(u, dim=0).max())
+        p = torch.zeros((1,) + tuple(u.shape[1:]), dtype=torch.float64, device=u.device)
         return p, u
 
     @property

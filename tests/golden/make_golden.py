@@ -234,7 +234,69 @@ def g7_turbulence():
     np.savez_compressed(os.path.join(OUT, "g7_decaying_turbulence.npz"), **out)
 
 
+def g8_obstacle_abb():
+    """lettuce's default Obstacle (lettuce/ext/_flows/obstacle.py:107-122):
+    equilibrium inlet, anti-bounce-back outlet, fullway bounce-back on the
+    mask; D2Q9 BGK 48 x 24 and D3Q27 KBC 16 x 8 x 8, FP64, 10 steps in
+    lettuce's default order (collide, then stream) and 10 pull steps."""
+    out = {}
+    cases = (("D2Q9", lt.D2Q9(), [48, 24], 4.0, "bgk"),
+             ("D3Q27", lt.D3Q27(), [16, 8, 8], 2.0, "kbc"))
+    for name, stencil, res, length_x, coll in cases:
+        for strat in (lt.StreamingStrategy.POST_STREAMING, lt.StreamingStrategy.PRE_STREAMING):
+            flow = lt.Obstacle(ctx(torch.float64), res, reynolds_number=50, mach_number=0.1,
+                               domain_length_x=length_x, stencil=stencil)
+            grid = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
+            if len(res) == 2:
+                mask = (grid[0] - 14) ** 2 + (grid[1] - 12) ** 2 < 4.5 ** 2
+            else:
+                mask = ((grid[0] - 5) ** 2 + (grid[1] - 3.5) ** 2 + (grid[2] - 4) ** 2
+                        < 2.5 ** 2)
+            flow.mask = mask
+            flow.initialize()
+            tau = flow.units.relaxation_parameter_lu
+            collision = lt.BGKCollision(tau) if coll == "bgk" else lt.KBCCollision()
+            sim = lt.Simulation(flow, collision, [], streaming_strategy=strat)
+            tag = "post" if strat == lt.StreamingStrategy.POST_STREAMING else "pre"
+            if tag == "post":
+                out[f"{name}_mask"] = mask
+                out[f"{name}_f0"] = flow.f.numpy().copy()
+                out[f"{name}_tau"] = np.array(tau)
+                out[f"{name}_ncm"] = sim.no_collision_mask.numpy()
+                out[f"{name}_nsm"] = sim.no_streaming_mask.numpy()
+            sim(10)
+            out[f"{name}_f10_{tag}"] = flow.f.numpy()
+    np.savez_compressed(os.path.join(OUT, "g8_obstacle_abb.npz"), **out)
+
+
+def g9_poiseuille_guo():
+    """lettuce's force test case (tests/collision/test_force.py:6-32):
+    PoiseuilleFlow2D 17^2, Re 1, Ma 0.02, zero start, BGK with the Guo force
+    (lettuce/ext/_force/guo.py:9-43), bounce-back walls, FP64, lettuce's
+    default order; the state after 10 and 100 steps and the velocity
+    u(acceleration) after 100 steps."""
+    flow = lt.PoiseuilleFlow2D(context=ctx(torch.float64), resolution=17, reynolds_number=1,
+                               mach_number=0.02, initialize_with_zeros=True)
+    acc = flow.units.convert_acceleration_to_lu(flow.acceleration)
+    tau = flow.units.relaxation_parameter_lu
+    force = lt.Guo(flow=flow, tau=tau, acceleration=acc)
+    sim = lt.Simulation(flow=flow, collision=lt.BGKCollision(tau=tau, force=force), reporter=[])
+    out = {"f0": flow.f.numpy().copy(), "tau": np.array(tau),
+           "acceleration_lu": acc.numpy(), "ncm": sim.no_collision_mask.numpy(),
+           "nsm": sim.no_streaming_mask.numpy()}
+    sim(10)
+    out["f10"] = flow.f.numpy().copy()
+    sim(90)
+    out["f100"] = flow.f.numpy()
+    out["u100"] = flow.u(acceleration=acc).numpy()
+    np.savez_compressed(os.path.join(OUT, "g9_poiseuille_guo.npz"), **out)
+
+
 if __name__ == "__main__":
-    for fn in (g1_spike, g2_tgv3d, g3_tgv2d, g4_kbc, g5_trt, g6_obstacle, g7_turbulence):
+    only = sys.argv[1:]
+    for fn in (g1_spike, g2_tgv3d, g3_tgv2d, g4_kbc, g5_trt, g6_obstacle, g7_turbulence,
+               g8_obstacle_abb, g9_poiseuille_guo):
+        if only and fn.__name__ not in only:
+            continue
         fn()
         print("wrote", fn.__name__)

@@ -77,3 +77,37 @@ def test_uniform_flow_passes_the_outlet_unchanged():
     sim(20)
     err = float((flow.f.cpu() - torch.from_numpy(f0)).abs().max())
     assert err <= 1e-14, err
+
+
+G8_CASES = {"D2Q9": (lt.D2Q9, [48, 24], 4.0, "bgk"), "D3Q27": (lt.D3Q27, [16, 8, 8], 2.0, "kbc")}
+
+
+@pytest.mark.parametrize("name", list(G8_CASES))
+@pytest.mark.parametrize("tag", ["post", "pre"])
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_default_obstacle_matches_lettuce(name, tag, dtype):
+    """lettuce's default Obstacle (equilibrium inlet, anti-bounce-back outlet,
+    fullway bounce-back; obstacle.py:107-122) against lettuce's own run
+    (golden g8): D2Q9 BGK and D3Q27 KBC, masks bit for bit, 10 steps in
+    either order; FP64 to 1e-12, FP32 to 1e-5 of the FP64 golden"""
+    g = golden("g8_obstacle_abb.npz")
+    stencil, res, length_x, coll = G8_CASES[name]
+    flow = lt.Obstacle(ctx(dtype), res, reynolds_number=50, mach_number=0.1,
+                       domain_length_x=length_x, stencil=stencil())
+    flow.mask = g[f"{name}_mask"]
+    flow.f = torch.from_numpy(g[f"{name}_f0"]).to(dtype)
+    tau = flow.units.relaxation_parameter_lu
+    assert tau == float(g[f"{name}_tau"])
+    collision = lt.BGKCollision(tau) if coll == "bgk" else lt.KBCCollision()
+    strategy = (lt.StreamingStrategy.POST_STREAMING if tag == "post"
+                else lt.StreamingStrategy.PRE_STREAMING)
+    sim = lt.Simulation(flow, collision, [], strategy)
+    assert np.array_equal(sim.no_collision_mask.cpu().numpy(), g[f"{name}_ncm"])
+    assert np.array_equal(sim.no_streaming_mask.cpu().numpy(), g[f"{name}_nsm"])
+    sim(10)
+    ref = g[f"{name}_f10_{tag}"]
+    got = flow.f.cpu().numpy().astype(np.float64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    elem = (np.abs(got - ref) / np.abs(ref)).max()
+    assert err <= (1e-12 if dtype == F64 else 1e-5), err
+    assert elem <= (1e-11 if dtype == F64 else 1e-4), elem

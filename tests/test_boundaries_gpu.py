@@ -1,0 +1,255 @@
+"""GPU parity of the boundary kernels beyond the 2D channel: the equilibrium
+boundary with spatially varying and broadcast values (per-cell FP64
+boundary field), every boundary on the 3D stencils, the masked KBC kernel,
+and lettuce's native no-streaming test, each against the oracle's
+restatement of lettuce's torch step (lettuce/_simulation.py:241-305)."""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+import lettuce_amd as lt
+from oracle import lettuce_torch as O
+from tests.test_oracle import _Grid, _perturbed
+from tests.test_parity_gpu import ArrayFlow, ctx, F32, F64
+
+pytestmark = pytest.mark.gpu
+
+STENCILS = {"D2Q9": (lt.D2Q9, O.D2Q9), "D3Q19": (lt.D3Q19, O.D3Q19),
+            "D3Q27": (lt.D3Q27, O.D3Q27)}
+
+
+def errs(got, ref):
+    """(max |got - ref| / max |ref|, max |got - ref| / |ref| entrywise)"""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    d = np.abs(got - ref)
+    return float(d.max() / np.abs(ref).max()), float((d / np.abs(ref)).max())
+
+
+def _broadcast_cases():
+    for name, (s, _) in STENCILS.items():
+        d = s().d
+        n = 12 if d == 2 else 6
+        for dims in itertools.product([1, n], repeat=d):
+            yield name, list(dims)
+
+
+@pytest.mark.parametrize("name,dims", list(_broadcast_cases()),
+                         ids=[f"{n}-{'x'.join(map(str, d))}" for n, d in _broadcast_cases()])
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+def test_equilibrium_boundary_broadcast_values_vs_oracle(name, dims, strategy):
+    """lettuce's native equilibrium-boundary test over every broadcast shape
+    {1, n}^d of the boundary values (tests/boundary/test_equilibrium_bc_pu.py:
+    125-164), with values that vary along the non-broadcast axes (lettuce's
+    generated kernel indexes them wrongly, so the reference semantics are
+    the torch path's, equilibrium_boundary_pu.py:23-69, :79-84): BGK, FP64,
+    3 steps, mask on the x = 0 plane and an interior block."""
+    ls, os_ = STENCILS[name]
+    d = ls().d
+    n = 12 if d == 2 else 6
+    res = [n] * d
+    g = torch.Generator().manual_seed(sum(dims) + 7 * d)
+    velocity = 0.02 + 0.05 * torch.rand([d] + dims, generator=g, dtype=F64)
+    pressure = 0.01 * (torch.rand([1] + dims, generator=g, dtype=F64) - 0.5)
+    mask = torch.zeros(res, dtype=torch.bool)
+    mask[0] = True
+    mask[tuple([slice(2, 4)] * d)] = True
+    f0 = _perturbed(os_(), n, F64, seed=3).f.numpy()
+    flow = ArrayFlow(ctx(F64), f0, ls())
+    flow._post = [lt.EquilibriumBoundaryPU(flow.context, flow, mask, velocity, pressure)]
+    sim = lt.Simulation(flow, lt.BGKCollision(0.6), [], getattr(lt.StreamingStrategy, strategy))
+    uniform = all(k == 1 for k in dims)
+    assert (sim.cell_classes.bc_field is None) == uniform
+    oflow = _Grid(os_(), res, F64, f0)
+    oflow.post_boundaries = [O.EquilibriumBoundaryPU(oflow.context, oflow, mask, velocity,
+                                                     pressure)]
+    osim = O.Simulation(oflow, O.BGKCollision(0.6), [], getattr(O.StreamingStrategy, strategy))
+    sim(3)
+    osim(3)
+    rel, elem = errs(flow.f.cpu().numpy(), oflow.f.numpy())
+    assert rel <= 1e-14 and elem <= 1e-13, (rel, elem)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_equilibrium_boundary_field_in_fp32_storage(dtype):
+    """a spatially varying boundary (per-cell FP64 field) in an FP32 and an
+    FP64 simulation, 20 pull steps on a D2Q9 channel-like grid"""
+    res = [24, 12]
+    g = torch.Generator().manual_seed(5)
+    velocity = torch.zeros([2, 1, 12], dtype=F64)
+    velocity[0, 0] = 0.05 * torch.sin(torch.linspace(0, np.pi, 12, dtype=F64))
+    pressure = 0.002 * torch.rand([1, 24, 12], generator=g, dtype=F64)
+    mask = torch.zeros(res, dtype=torch.bool)
+    mask[0] = True
+    f0 = _perturbed(O.D2Q9(), 12, F64, seed=9).f.numpy()
+    f0 = np.concatenate([f0, f0], axis=1)
+    flow = ArrayFlow(ctx(dtype), f0, lt.D2Q9())
+    flow._post = [lt.EquilibriumBoundaryPU(flow.context, flow, mask, velocity, pressure)]
+    sim = lt.Simulation(flow, lt.BGKCollision(0.7), [], lt.StreamingStrategy.PRE_STREAMING)
+    oflow = _Grid(O.D2Q9(), res, dtype, f0)
+    oflow.post_boundaries = [O.EquilibriumBoundaryPU(oflow.context, oflow, mask,
+                                                     velocity.to(dtype), pressure.to(dtype))]
+    osim = O.Simulation(oflow, O.BGKCollision(0.7), [], O.StreamingStrategy.PRE_STREAMING)
+    sim(20)
+    osim(20)
+    rel, elem = errs(flow.f.cpu().numpy(), oflow.f.numpy())
+    if dtype == F64:
+        assert rel <= 1e-14 and elem <= 1e-13, (rel, elem)
+    else:
+        assert rel <= 1e-6 and elem <= 1e-5, (rel, elem)
+
+
+def _sphere(res, centre, radius):
+    grid = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
+    return sum((g - c) ** 2 for g, c in zip(grid, centre)) < radius ** 2
+
+
+class _Channel3D(O.Flow):
+    """the oracle flow with lt.Obstacle's units and boundaries: equilibrium
+    inlet at x = 0, an equilibrium or anti-bounce-back outlet at x = -1, and
+    a fullway bounce-back mask (lettuce's Obstacle,
+    lettuce/ext/_flows/obstacle.py:107-122)"""
+
+    def __init__(self, stencil, res, length_x, f0, mask, outlet, dtype):
+        units = O.UnitConversion(50, 0.1, characteristic_length_lu=res[0] / length_x,
+                                 characteristic_length_pu=1, characteristic_velocity_pu=1)
+        super().__init__(O.Context("cpu", dtype), res, units, stencil)
+        self.f = torch.as_tensor(f0).to(dtype).clone()
+        x = torch.meshgrid(*[units.convert_length_to_pu(torch.arange(n)) for n in res],
+                           indexing="ij")[0]
+        unit = torch.eye(len(res))[0]
+        direction = [int(c) for c in unit.tolist()]
+        out = (O.EquilibriumOutletP(direction, self) if outlet == "equilibrium"
+               else O.AntiBounceBackOutlet(direction, self))
+        self.post_boundaries = [
+            O.EquilibriumBoundaryPU(self.context, self, torch.abs(x) < 1e-6, unit),
+            out, O.BounceBackBoundary(torch.as_tensor(mask))]
+
+
+@pytest.mark.parametrize("name,coll", [("D3Q19", "bgk"), ("D3Q19", "trt"), ("D3Q27", "bgk"),
+                                       ("D3Q27", "kbc")])
+@pytest.mark.parametrize("outlet", ["equilibrium", "antibounceback"])
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_obstacle3d_fullway_vs_oracle(name, coll, outlet, strategy, dtype):
+    """3D Obstacle (20 x 10 x 8, a sphere): equilibrium inlet, equilibrium or
+    anti-bounce-back outlet, fullway bounce-back; 40 steps against the
+    oracle: FP64 to 1e-12, FP32 to 1e-5 (max-normalised) and 1e-4
+    (entrywise)"""
+    ls, os_ = STENCILS[name]
+    res = [20, 10, 8]
+    mask = _sphere(res, (6, 4.5, 3.5), 2.6)
+    flow = lt.Obstacle(ctx(dtype), res, reynolds_number=50, mach_number=0.1,
+                       domain_length_x=4.0, stencil=ls(), outlet=outlet)
+    flow.mask = mask
+    flow.initialize()
+    f0 = flow.f.cpu().numpy().astype(np.float64)
+    tau = flow.units.relaxation_parameter_lu
+    mk = {"bgk": (lambda: lt.BGKCollision(tau), lambda: O.BGKCollision(tau)),
+          "trt": (lambda: lt.TRTCollision(tau, 0.8), lambda: O.TRTCollision(tau, 0.8)),
+          "kbc": (lambda: lt.KBCCollision(), lambda: O.KBCCollision())}[coll]
+    sim = lt.Simulation(flow, mk[0](), [], getattr(lt.StreamingStrategy, strategy))
+    oflow = _Channel3D(os_(), res, 4.0, f0, mask, outlet, dtype)
+    osim = O.Simulation(oflow, mk[1](), [], getattr(O.StreamingStrategy, strategy))
+    assert np.array_equal(sim.no_collision_mask.cpu().numpy(), osim.no_collision_mask.numpy())
+    assert np.array_equal(sim.no_streaming_mask.cpu().numpy(), osim.no_streaming_mask.numpy())
+    sim(40)
+    osim(40)
+    rel, elem = errs(flow.f.cpu().numpy(), oflow.f.numpy())
+    if dtype == F64:
+        assert rel <= 1e-12 and elem <= 1e-11, (rel, elem)
+    else:
+        assert rel <= 1e-5 and elem <= 1e-4, (rel, elem)
+
+
+@pytest.mark.parametrize("name,coll", [("D3Q19", "bgk"), ("D3Q27", "kbc"), ("D3Q27", "bgk")])
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_channel3d_halfway_vs_oracle(name, coll, strategy, dtype):
+    """3D channel (24 x 12 x 10) with a sphere: equilibrium inlet, equilibrium
+    outlet, halfway bounce-back on the sphere and on the four side walls;
+    40 steps against the oracle, and the momentum-exchange force on the
+    solids against the oracle's link sum"""
+    ls, os_ = STENCILS[name]
+    res = [24, 12, 10]
+    mask = _sphere(res, (7, 5.5, 4.5), 2.8)
+    oflow = O.ObstacleChannel(O.Context("cpu", dtype), res, 100, 0.1, domain_length_x=2.4,
+                              mask=mask, stencil=os_())
+    f0 = oflow.f.numpy().copy()
+    tau = oflow.units.relaxation_parameter_lu
+    mk = {"bgk": (lambda: lt.BGKCollision(tau), lambda: O.BGKCollision(tau)),
+          "kbc": (lambda: lt.KBCCollision(), lambda: O.KBCCollision())}[coll]
+    osim = O.Simulation(oflow, mk[1](), [], getattr(O.StreamingStrategy, strategy))
+    flow = lt.Obstacle(ctx(dtype), res, 100, 0.1, domain_length_x=2.4, stencil=ls(),
+                       outlet="equilibrium", solid="halfway", walls=True)
+    flow.mask = mask
+    flow.f = torch.from_numpy(f0)
+    assert flow.units.relaxation_parameter_lu == tau
+    sim = lt.Simulation(flow, mk[0](), [], getattr(lt.StreamingStrategy, strategy))
+    sim(40)
+    osim(40)
+    rel, elem = errs(flow.f.cpu().numpy(), oflow.f.numpy())
+    if dtype == F64:
+        assert rel <= 1e-12 and elem <= 1e-11, (rel, elem)
+    else:
+        assert rel <= 1e-5 and elem <= 1e-4, (rel, elem)
+    if strategy == "PRE_STREAMING" and dtype == F64:
+        # the FP64 link force of the stored post-collision state against
+        # 2 sum f_d(x) e_d over the same links, with the oracle's populations
+        links = sim.halfway_links()
+        F = flow.post_streaming_boundaries[0].force(sim).cpu().numpy()
+        q = ls().q
+        e = torch.tensor(os_.e, dtype=F64)
+        cells, dirs = links[:, 0].long(), links[:, 1].long()
+        vals = oflow.f.to(F64).reshape(q, -1)[dirs, cells]
+        ref = (2 * vals[:, None] * e[dirs]).sum(0).numpy()
+        assert len(links) > 0
+        assert np.abs(F - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+def test_masked_kbc_channel_cylinder_vs_oracle(dtype, strategy):
+    """the masked KBC kernel (the plain-KBC leg of the channel benchmark):
+    D2Q9 channel with a cylinder, equilibrium inlet and outlet, halfway
+    bounce-back on the cylinder and the walls, KBC; 200 steps FP64, 40 FP32
+    (KBC's gamma amplifies rounding differences over long FP32 runs)"""
+    nx, ny = 64, 32
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+    mask = (xx - 16) ** 2 + (yy - 15.5) ** 2 < 5.0 ** 2
+    oflow = O.ObstacleChannel(O.Context("cpu", dtype), [nx, ny], 100, 0.1,
+                              domain_length_x=nx / 10.0, mask=mask, stencil=O.D2Q9())
+    f0 = oflow.f.numpy().copy()
+    osim = O.Simulation(oflow, O.KBCCollision(), [], getattr(O.StreamingStrategy, strategy))
+    flow = lt.Obstacle(ctx(dtype), [nx, ny], 100, 0.1, domain_length_x=nx / 10.0,
+                       stencil=lt.D2Q9(), outlet="equilibrium", solid="halfway", walls=True)
+    flow.mask = mask
+    flow.f = torch.from_numpy(f0)
+    sim = lt.Simulation(flow, lt.KBCCollision(), [], getattr(lt.StreamingStrategy, strategy))
+    steps = 200 if dtype == F64 else 40
+    sim(steps)
+    osim(steps)
+    rel, elem = errs(flow.f.cpu().numpy(), oflow.f.numpy())
+    if dtype == F64:
+        assert rel <= 1e-11 and elem <= 1e-10, (rel, elem)
+    else:
+        assert rel <= 1e-5 and elem <= 1e-4, (rel, elem)
+
+
+@pytest.mark.parametrize("name", ["D2Q9", "D3Q19", "D3Q27"])
+def test_no_collision_periodic_run_returns_to_the_start(name):
+    """lettuce's tests/native/test_native_no_streaming_mask.py:4-22: without
+    collision, 64 post-streaming steps on a 16-periodic grid move every
+    population by a multiple of the period, so f returns to its start (bit
+    for bit); assigning simulation.no_streaming_mask afterwards changes
+    nothing, as in lettuce's native path"""
+    ls, os_ = STENCILS[name]
+    n = 16 if ls().d == 2 else 8
+    f0 = _perturbed(os_(), n, F32, seed=2).f.numpy()
+    flow = ArrayFlow(ctx(F32), f0, ls())
+    sim = lt.Simulation(flow, lt.NoCollision(), [])
+    sim.no_streaming_mask = torch.zeros(flow.resolution, dtype=torch.bool)
+    sim(64)
+    assert np.array_equal(flow.f.cpu().numpy(), f0)

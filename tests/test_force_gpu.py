@@ -73,3 +73,27 @@ def test_poiseuille_force_like_lettuce():
         assert a.max() == pytest.approx(b.max(), rel=0.01)
         assert a == pytest.approx(b, rel=None, abs=0.01 * float(u_ref[0].max()))
     assert len(errors.out) == 11 and errors.out[-1][0] < errors.out[1][0]
+
+
+def test_poiseuille_guo_matches_lettuce_trajectory():
+    """lettuce's force test case run by lettuce itself (golden g9):
+    PoiseuilleFlow2D 17^2 with the Guo force and bounce-back walls, FP64,
+    post-streaming; the state after 10 and 100 steps and the velocity with
+    the a / (2 rho) shift"""
+    from tests.conftest import golden
+    g = golden("g9_poiseuille_guo.npz")
+    flow = lt.PoiseuilleFlow2D(lt.Context("cuda:0", dtype=torch.float64), resolution=17,
+                               reynolds_number=1, mach_number=0.02,
+                               initialize_with_zeros=True)
+    assert np.array_equal(flow.f.cpu().numpy(), g["f0"])
+    acc = flow.units.convert_acceleration_to_lu(flow.acceleration)
+    tau = flow.units.relaxation_parameter_lu
+    sim = lt.Simulation(flow, lt.BGKCollision(tau, force=lt.Guo(flow, tau, acc)), [])
+    assert np.array_equal(sim.no_collision_mask.cpu().numpy(), g["ncm"])
+    assert np.array_equal(sim.no_streaming_mask.cpu().numpy(), g["nsm"])
+    sim(10)
+    assert np.abs(flow.f.cpu().numpy() - g["f10"]).max() <= 1e-14
+    sim(90)
+    assert np.abs(flow.f.cpu().numpy() - g["f100"]).max() <= 1e-14
+    u = flow.u(acceleration=acc).cpu().numpy()
+    assert np.abs(u - g["u100"]).max() <= 1e-15

@@ -293,3 +293,66 @@ def test_grid_limits_apply_per_i0_plane_without_a_gpu():
                         (2 * 40000 * 30000, 40000 * 30000, 30000, 1), mask=True)
     assert lib.lbm_run_plan(ctypes.byref(masked), 1, plan) == -1
     assert b"2^31 - 1 cells" in lib.lbm_last_error()
+
+
+def test_high_ma_reporter_counts_non_finite_velocity(monkeypatch):
+    """HighMaReporter fails on cells whose |u|^2 is not finite (scan count 1)
+    even with no NaN population and a finite max |u|^2 below the threshold"""
+    from lettuce_amd.ext import _failure
+    sim = SimpleNamespace(flow=SimpleNamespace(stencil=lt.D2Q9()))
+    rep = lt.HighMaReporter(1, threshold=0.3)
+    for scan, failed in (([0.0, 0.0, 0.01], False), ([0.0, 1.0, 0.01], True),
+                         ([2.0, 0.0, 0.01], True), ([0.0, 0.0, 0.5], True)):
+        monkeypatch.setattr(_failure, "_scan", lambda flow, s=scan: torch.tensor(s))
+        assert rep.is_failed(sim) is failed
+
+
+@pytest.mark.parametrize("res,key", [([32, 32], "u0_lu"), ([8, 8, 8], "u0_lu_3d")])
+def test_decaying_turbulence_field_matches_lettuce(res, key):
+    """The package's DecayingTurbulence field (torch FP64 FFTs) against the
+    field lettuce itself generated (golden g7: Re 1000, Ma 0.1, k0 4, seed
+    0), in lattice units, and lettuce's characteristic velocity max |u|"""
+    from lettuce_amd.ext._flows import turbulence_velocity
+    g = golden("g7_decaying_turbulence.npz")
+    units = lt.UnitConversion(reynolds_number=1000, mach_number=0.1,
+                              characteristic_length_lu=res[0],
+                              characteristic_length_pu=2 * np.pi,
+                              characteristic_velocity_pu=None)
+    u, _, _ = turbulence_velocity(res, 4, 0.5, units.convert_length_to_pu(1.0),
+                                  units.characteristic_length_pu, 0, "cpu")
+    units.characteristic_velocity_pu = float(torch.linalg.vector_norm(u, dim=0).max())
+    u_lu = units.convert_velocity_to_lu(u).numpy()
+    ref = g[key]
+    assert np.abs(u_lu - ref).max() <= 1e-12 * np.abs(ref).max()
+    if key == "u0_lu":
+        assert abs(units.characteristic_velocity_pu / float(g["char_velocity_pu"]) - 1) <= 1e-13
+
+
+@pytest.mark.parametrize("res", [[16, 16], [8, 8, 8]])
+def test_decaying_turbulence_spectrum_is_the_shell_sum(res):
+    """the shell spectrum (a scatter-add over shell indices) against a
+    brute-force sum over each shell k - 1/2 < |k| <= k + 1/2 of
+    E(k) ~ k^4 exp(-2 (k/4)^2), normalised to the initial energy"""
+    from lettuce_amd.ext._flows import turbulence_velocity
+    _, spec, wn = turbulence_velocity(res, 4, 0.5, 2 * np.pi / res[0], 2 * np.pi, 0, "cpu")
+    k = np.stack(np.meshgrid(*[np.fft.fftfreq(n, 1 / n) for n in res], indexing="ij"))
+    norm = np.sqrt((k ** 2).sum(0))
+    ek = norm ** 4 * np.exp(-2 * (norm / 4) ** 2)
+    ek = 0.5 * ek / ek.sum()
+    assert len(wn) == int(norm.max()) and list(wn) == list(range(len(wn)))
+    for s in wn:
+        shell = (norm > s - 0.5) & (norm <= s + 0.5)
+        assert abs(spec[s] - ek[shell].sum()) <= 1e-14
+    assert abs(spec.sum() - ek[norm <= len(wn) - 0.5].sum()) <= 1e-14
+
+
+def test_poiseuille_units_and_acceleration_match_lettuce():
+    """PoiseuilleFlow2D's units (tau) and the acceleration in lattice units
+    against lettuce's (golden g9: 17^2, Re 1, Ma 0.02)"""
+    g = golden("g9_poiseuille_guo.npz")
+    units = lt.UnitConversion(reynolds_number=1, mach_number=0.02,
+                              characteristic_length_lu=16, characteristic_length_pu=1,
+                              characteristic_velocity_pu=1)
+    assert units.relaxation_parameter_lu == float(g["tau"])
+    a = units.convert_acceleration_to_lu(torch.tensor([0.001, 0.0], dtype=torch.float64))
+    assert np.array_equal(np.asarray(a), g["acceleration_lu"])

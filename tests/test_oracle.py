@@ -284,6 +284,87 @@ def test_golden_decaying_turbulence_field():
     assert np.allclose(flow3.u().numpy(), g["u0_lu_3d"], rtol=0, atol=1e-14)
 
 
+class _DefaultObstacle(O.Flow):
+    """lettuce's default Obstacle (lettuce/ext/_flows/obstacle.py:107-122):
+    equilibrium inlet at x = 0, anti-bounce-back outlet at x = -1, fullway
+    bounce-back on the mask; Re 50, Ma 0.1, characteristic length
+    res[0] / domain_length_x lattice units"""
+
+    def __init__(self, stencil, res, length_x, f0, mask, dtype=F64):
+        units = O.UnitConversion(50, 0.1, characteristic_length_lu=res[0] / length_x,
+                                 characteristic_length_pu=1, characteristic_velocity_pu=1)
+        super().__init__(O.Context("cpu", dtype), res, units, stencil)
+        self.f = torch.as_tensor(f0).to(dtype).clone()
+        x = torch.meshgrid(*[units.convert_length_to_pu(torch.arange(n)) for n in res],
+                           indexing="ij")[0]
+        unit = torch.eye(len(res))[0]
+        self.post_boundaries = [
+            O.EquilibriumBoundaryPU(self.context, self, torch.abs(x) < 1e-6, unit),
+            O.AntiBounceBackOutlet([int(c) for c in unit.tolist()], self),
+            O.BounceBackBoundary(torch.as_tensor(mask)),
+        ]
+
+
+G8_CASES = {"D2Q9": (O.D2Q9, [48, 24], 4.0, "bgk"), "D3Q27": (O.D3Q27, [16, 8, 8], 2.0, "kbc")}
+
+
+@pytest.mark.parametrize("name", list(G8_CASES))
+@pytest.mark.parametrize("tag", ["post", "pre"])
+def test_golden_default_obstacle_abb(name, tag):
+    """the oracle's Obstacle with the anti-bounce-back outlet against
+    lettuce's own (golden g8): masks bit for bit, 10 steps in FP64"""
+    g = golden("g8_obstacle_abb.npz")
+    stencil, res, length_x, coll = G8_CASES[name]
+    flow = _DefaultObstacle(stencil(), res, length_x, g[f"{name}_f0"], g[f"{name}_mask"])
+    tau = float(g[f"{name}_tau"])
+    assert tau == flow.units.relaxation_parameter_lu
+    collision = O.BGKCollision(tau) if coll == "bgk" else O.KBCCollision()
+    strategy = O.StreamingStrategy.POST_STREAMING if tag == "post" else \
+        O.StreamingStrategy.PRE_STREAMING
+    sim = O.Simulation(flow, collision, [], strategy)
+    assert np.array_equal(sim.no_collision_mask.numpy(), g[f"{name}_ncm"])
+    assert np.array_equal(sim.no_streaming_mask.numpy(), g[f"{name}_nsm"])
+    sim(10)
+    ref = g[f"{name}_f10_{tag}"]
+    assert np.abs(flow.f.numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+class _Poiseuille(O.Flow):
+    """lettuce's PoiseuilleFlow2D (lettuce/ext/_flows/poiseuille.py:18-86):
+    units with characteristic length res - 1, bounce-back on the first and
+    last row of y"""
+
+    def __init__(self, g, n=17):
+        units = O.UnitConversion(1, 0.02, characteristic_length_lu=n - 1,
+                                 characteristic_length_pu=1, characteristic_velocity_pu=1)
+        super().__init__(O.Context("cpu", F64), [n, n], units, O.D2Q9())
+        self.f = torch.from_numpy(g["f0"]).clone()
+        mask = torch.zeros(n, n, dtype=torch.bool)
+        mask[:, [0, -1]] = True
+        self.post_boundaries = [O.BounceBackBoundary(mask)]
+
+
+def test_golden_poiseuille_guo():
+    """lettuce's force test case (golden g9): BGK with the Guo force and
+    bounce-back walls, FP64, post-streaming, 10 and 100 steps, and the
+    velocity with the a / (2 rho) shift"""
+    g = golden("g9_poiseuille_guo.npz")
+    flow = _Poiseuille(g)
+    tau = float(g["tau"])
+    assert tau == flow.units.relaxation_parameter_lu
+    a = torch.from_numpy(g["acceleration_lu"])
+    sim = O.Simulation(flow, O.BGKCollision(tau, force=O.Guo(flow, tau, a)), [])
+    assert np.array_equal(sim.no_collision_mask.numpy(), g["ncm"])
+    assert np.array_equal(sim.no_streaming_mask.numpy(), g["nsm"])
+    sim(10)
+    assert np.abs(flow.f.numpy() - g["f10"]).max() <= 1e-14
+    sim(90)
+    assert np.abs(flow.f.numpy() - g["f100"]).max() <= 1e-14
+    rho = flow.rho()
+    u = flow.j() / rho + 0.5 * a.reshape(2, 1, 1) / rho
+    assert np.abs(u.numpy() - g["u100"]).max() <= 1e-15
+
+
 # ------------------------------------------------------ halfway bounce-back
 def _channel(n, ny, dtype=F64):
     s = O.D2Q9()
