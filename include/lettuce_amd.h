@@ -250,6 +250,22 @@ int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3],
                    const uint8_t* mask, double* out, void* workspace,
                    void* stream);
 
+/* lbm_reduce_f64 without the workspace argument; it allocates nothing.  No
+ * lettuce counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-block partial sums go to a static device array of
+ * the library, LBM_REDUCE_STATIC_BYTES bytes on each device it runs on,
+ * held until the process ends; a grid whose reduction needs more partials
+ * is rejected with LBM_E_ARG (none does: the partials are capped at 2048
+ * blocks, lbm_reduce_workspace_bytes).  It runs on the device current on
+ * the calling thread, which must own f, mask, out and stream.  Calls on one
+ * device share the array: calls in one stream are ordered, calls that
+ * overlap on one device (other streams, other threads) race. */
+#define LBM_REDUCE_STATIC_BYTES 81920
+int lbm_reduce_f64_static(int32_t stencil, int32_t storage, const int64_t n[3],
+                          const int64_t stride[4], const void* f, double rho0_shift,
+                          const uint8_t* mask, double* out, void* stream);
+
 /* Failure scan over all cells, the checks of NaNReporter.is_failed
  * (torch.isnan(flow.f).any(), lettuce/ext/_reporter/failure_reporter.py:144-145)
  * and HighMaReporter.is_failed (|u| / cs > threshold, :168-171) in one pass:

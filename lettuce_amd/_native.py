@@ -109,6 +109,8 @@ _SIGNATURES = {
     "lbm_slab_step": ([ctypes.POINTER(LbmStepArgs), _i32, _vp, _vp, _vp, _vp, _vp],
                       ctypes.c_int),
     "lbm_set_tuning": ([ctypes.c_char_p, _i32], ctypes.c_int),
+    "lbm_reduce_f64_static": ([_i32, _i32, _I64_3, _I64_4, _vp, _dbl, _vp, _vp, _vp],
+                              ctypes.c_int),
 }
 
 _lock = threading.Lock()

@@ -8,7 +8,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "lbm_step_impl.h"
@@ -994,6 +996,48 @@ double shift_for(int storage, double rho0) {
 
 }  // namespace
 
+// Per-block partial sums of lbm_reduce_f64_static: a device array of the
+// library in place of the caller's workspace, RED_MAX_BLOCKS partials of
+// RED_N doubles (81,920 bytes) on every device the library runs on, held
+// until the process ends.  Host code must not use this name as a device
+// address: each device holds its own copy, whose address
+// static_scratch_of_current_device returns.
+static __device__ __align__(256) double scratch[RED_MAX_BLOCKS * RED_N];
+static_assert(sizeof(scratch) == LBM_REDUCE_STATIC_BYTES, "header size of the static scratch");
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 256;
+std::atomic<double*> g_scratch[SCRATCH_MAX_DEVICES];
+std::mutex g_scratch_mutex;
+
+// The address of `scratch` on the device current on this thread, looked up
+// with hipGetSymbolAddress on the first call per device and kept.
+int static_scratch_of_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LBM_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(LBM_E_UNSUPPORTED, "device %d: the static scratch tracks devices 0..%d", dev,
+                SCRATCH_MAX_DEVICES - 1);
+  double* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void* addr = nullptr;
+      e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !addr)
+        return fail(LBM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                    hipGetErrorString(e));
+      p = (double*)addr;
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return LBM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int lbm_equilibrium(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
@@ -1046,6 +1090,25 @@ int lbm_reduce_f64(int32_t stencil, int32_t storage, const int64_t n[3], const i
   return with_lattice<RedRun>(stencil, storage, make_grid(n, stride), f,
                               shift_for(storage, rho0_shift), mask, out, (double*)workspace,
                               (hipStream_t)stream);
+}
+
+int lbm_reduce_f64_static(int32_t stencil, int32_t storage, const int64_t n[3],
+                          const int64_t stride[4], const void* f, double rho0_shift,
+                          const uint8_t* mask, double* out, void* stream) {
+  int rc = check_layout(stencil, storage, n, stride);
+  if (rc) return rc;
+  if (!f || !out) return fail(LBM_E_ARG, "NULL pointer");
+  // the partials lbm_reduce_f64 would need; every block writes its own
+  // before the final sum reads them, so nothing must start at zero
+  const int64_t need = lbm_reduce_workspace_bytes(n);
+  if (need > (int64_t)sizeof(scratch))
+    return fail(LBM_E_ARG, "the reduction needs %lld bytes of partial sums, above the %lld of the static scratch",
+                (long long)need, (long long)sizeof(scratch));
+  double* ws = nullptr;
+  rc = static_scratch_of_current_device(&ws);
+  if (rc) return rc;
+  return with_lattice<RedRun>(stencil, storage, make_grid(n, stride), f,
+                              shift_for(storage, rho0_shift), mask, out, ws, (hipStream_t)stream);
 }
 
 int lbm_scan_f64(int32_t stencil, int32_t storage, const int64_t n[3], const int64_t stride[4],
