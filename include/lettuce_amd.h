@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 /* error codes */
 #define LBM_OK 0
@@ -316,15 +316,18 @@ int lbm_learned_pack(const float* w1, const float* b1, const float* w2,
  * trainable collision; these are the forward and adjoint kernels behind
  * lettuce_amd.ext.LearnedRollout, a torch.autograd.Function).
  *
- * Periodic D2Q9 with pull streaming, a batch of n[0] independent n[1] x n[2]
- * fields, populations contiguous as [9][n0][n1][n2] in F32 or F64 (storage),
- * and the MLP as one vector `params` of the same type:
+ * Periodic pull streaming in F32 or F64 (storage), the MLP as one vector
+ * `params` of the same type:
  *   [w1 (32x6), b1 (32), w2 (32x32), b2 (32), w3 (3x32), b3 (3)]
+ *   stencil LBM_D2Q9   a batch of n[0] independent n[1] x n[2] fields,
+ *                      populations contiguous as [9][n0][n1][n2];
+ *   stencil LBM_D3Q27  one n[0] x n[1] x n[2] field, [27][n0][n1][n2].
  * The forward step is the learned collision's formula with the MLP in the
  * arithmetic type (no bf16 rounding) and without the KBC fallback:
  *   f' = g - ds/tau - sum_k clamp(1/tau + o_k, rate_lo, rate_hi) m_k(dh) phi_k
- * with g the pulled populations.  The clamp passes the gradient for raw
- * rates in [rate_lo, rate_hi] and blocks it outside.
+ * with g the pulled populations (D3Q27: the rates of the raw moments of dh
+ * of orders 3, 4 and 5-6).  The clamp passes the gradient for raw rates in
+ * [rate_lo, rate_hi] and blocks it outside.
  * ------------------------------------------------------------------------- */
 #define LBM_LEARNED_PARAMS 1379
 /* per-cell record of the backward kernel: x (6), a1 (32), a2 (32),
@@ -333,8 +336,8 @@ int lbm_learned_pack(const float* w1, const float* b1, const float* w2,
 #define LBM_LEARNED_RECORD 137
 
 /* f_out = learned collide(pull(f_in)) */
-int lbm_learned_train_forward(int32_t storage, const int64_t n[3], double tau,
-                              const void* params, double in_scale,
+int lbm_learned_train_forward(int32_t stencil, int32_t storage, const int64_t n[3],
+                              double tau, const void* params, double in_scale,
                               double rate_lo, double rate_hi,
                               const void* f_in, void* f_out, void* stream);
 
@@ -344,8 +347,8 @@ int lbm_learned_train_forward(int32_t storage, const int64_t n[3], double tau,
  * (lam_gathered != 0), the transpose of pull streaming.  Writes
  * mu = (d collide / d g)^T lam and, if record is not null, the per-cell
  * record (LBM_LEARNED_RECORD x cells values of the storage type). */
-int lbm_learned_train_backward(int32_t storage, const int64_t n[3], double tau,
-                               const void* params, double in_scale,
+int lbm_learned_train_backward(int32_t stencil, int32_t storage, const int64_t n[3],
+                               double tau, const void* params, double in_scale,
                                double rate_lo, double rate_hi,
                                const void* f_in, const void* lam,
                                int32_t lam_gathered, void* mu, void* record,
@@ -353,8 +356,8 @@ int lbm_learned_train_backward(int32_t storage, const int64_t n[3], double tau,
 
 /* lam_q(x) = mu_q(x + e_q): the adjoint of periodic pull streaming (the
  * gradient with respect to the populations before the step) */
-int lbm_unpull(int32_t storage, const int64_t n[3], const void* mu, void* lam,
-               void* stream);
+int lbm_unpull(int32_t stencil, int32_t storage, const int64_t n[3], const void* mu,
+               void* lam, void* stream);
 
 /* ---------------------------------------------------------------------------
  * x-slabs: one device's share of a lattice split along i0 (no lettuce

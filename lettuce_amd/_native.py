@@ -21,7 +21,7 @@ __all__ = ["lib", "LbmStepArgs", "LbmCellClass", "LbmLearned", "check",
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "liblettuce_amd.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # enum values of include/lettuce_amd.h
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
@@ -99,11 +99,11 @@ _SIGNATURES = {
                             _vp, _vp, _vp], ctypes.c_int),
     "lbm_learned_pack": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_stream_copy": ([_vp, _vp, _i64, _i32, _vp], ctypes.c_int),
-    "lbm_learned_train_forward": ([_i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp, _vp],
-                                  ctypes.c_int),
-    "lbm_learned_train_backward": ([_i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp, _i32,
-                                    _vp, _vp, _vp], ctypes.c_int),
-    "lbm_unpull": ([_i32, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_learned_train_forward": ([_i32, _i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp,
+                                   _vp], ctypes.c_int),
+    "lbm_learned_train_backward": ([_i32, _i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp,
+                                    _i32, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_unpull": ([_i32, _i32, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_halo_planes": ([_i32], _i64),
     "lbm_halo_pack": ([_i32, _i32, _I64_3, _I64_4, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_slab_step": ([ctypes.POINTER(LbmStepArgs), _i32, _vp, _vp, _vp, _vp, _vp],

@@ -1,8 +1,8 @@
-// Training kernels of the D2Q9 learned collision: the forward step with
-// the MLP evaluated in the arithmetic type (FP32, or FP64 for gradient
-// checks) and its hand-written adjoint.
+// Training kernels of the learned collision (D2Q9 and D3Q27): the forward
+// step with the MLP evaluated in the arithmetic type (FP32, or FP64 for
+// gradient checks) and its hand-written adjoint.
 //
-// Forward per cell (g = pulled populations, s = in_scale):
+// D2Q9 forward per cell (g = pulled populations, s = in_scale):
 //   rho = sum g, u = j / rho, feq, df = g - feq
 //   ds = KBC shear part of df (T = xx + yy, N = xx - yy, Pxy), dh = df - ds
 //   x  = s / rho (T, N, Pxy, xxy, xyy, xxyy)(df)
@@ -11,6 +11,15 @@
 //   f' = g - ds / tau - sum_k r_k m_k(dh) phi_k
 // (the inference kernel's formula without bf16 rounding and without the
 // KBC fallback; the clamp has zero derivative outside [rate_lo, rate_hi]).
+//
+// D3Q27 forward per cell, in the raw-moment basis t = M g (moments
+// x^a y^b z^c, a, b, c <= 2, the inference kernel's formulation):
+//   rho = t_000, u = (t_100, t_010, t_001) / rho
+//   t  <- t - M feq(rho, u) on orders >= 2 (closed form), so t = M df there
+//   x  = s / rho (t200 + t020 + t002, t200 - t002, t020 - t002, t110, t101, t011)
+//   the MLP and the clamped rates r as for D2Q9
+//   f' = g - Phi(w . t), Phi = M^-1, w = 1/tau on order 2, r_0 / r_1 / r_2
+//        on orders 3 / 4 / 5-6, 0 below
 //
 // Adjoint of one step f_{n+1} = C(S f_n), S the periodic pull:
 //   mu(x)         = (dC/dg)^T lambda_{n+1}(x)        per cell, analytic
@@ -58,6 +67,318 @@ __device__ __forceinline__ C sigmoid(C z) {
   return C(1) / (C(1) + exp(-z));
 }
 
+// z1 = W1 x + b1, a1 = silu(z1), z2 = W2 a1 + b2, a2 = silu(z2),
+// raw = 1/tau + W3 a2 + b3, r = clamp(raw); with rec set, x, a1 and a2 go to
+// the record (stride rstride)
+template <class C>
+__device__ __forceinline__ void mlp_forward(const C* __restrict__ P, const C (&x)[6], C tau_inv,
+                                            C lo, C hi, C (&z1)[HID], C (&z2)[HID], C (&raw)[3],
+                                            C (&r)[3], C* rec, int64_t rstride) {
+  C a[HID];
+#pragma unroll
+  for (int i = 0; i < HID; ++i) {
+    C z = P[P_B1 + i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) z += P[P_W1 + i * 6 + k] * x[k];
+    z1[i] = z;
+    a[i] = z * sigmoid(z);
+  }
+  if (rec) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec[(R_X + k) * rstride] = x[k];
+#pragma unroll
+    for (int i = 0; i < HID; ++i) rec[(R_A1 + i) * rstride] = a[i];
+  }
+#pragma unroll
+  for (int i = 0; i < HID; ++i) {
+    C z = P[P_B2 + i];
+#pragma unroll
+    for (int j = 0; j < HID; ++j) z += P[P_W2 + i * HID + j] * a[j];
+    z2[i] = z;
+  }
+#pragma unroll
+  for (int i = 0; i < HID; ++i) a[i] = z2[i] * sigmoid(z2[i]);
+  if (rec) {
+#pragma unroll
+    for (int i = 0; i < HID; ++i) rec[(R_A2 + i) * rstride] = a[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    C o = P[P_B3 + k];
+#pragma unroll
+    for (int j = 0; j < HID; ++j) o += P[P_W3 + k * HID + j] * a[j];
+    raw[k] = tau_inv + o;
+    r[k] = raw[k] < lo ? lo : (raw[k] > hi ? hi : raw[k]);
+  }
+}
+
+// silu'(z) = sigma(z) (1 + z (1 - sigma(z)))
+template <class C>
+__device__ __forceinline__ C dsilu(C z) {
+  const C sg = sigmoid(z);
+  return sg * (C(1) + z * (C(1) - sg));
+}
+
+// dL/dx from dL/do (go); z2 is overwritten by dL/dz2; with rec set, dz1,
+// dz2 and do go to the record
+template <class C>
+__device__ __forceinline__ void mlp_backward(const C* __restrict__ P, const C (&go)[3],
+                                             const C (&z1)[HID], C (&z2)[HID], C (&gx)[6],
+                                             C* rec, int64_t rstride) {
+#pragma unroll
+  for (int i = 0; i < HID; ++i) {
+    C v = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v += P[P_W3 + k * HID + i] * go[k];
+    z2[i] = v * dsilu(z2[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) gx[k] = 0;
+#pragma unroll
+  for (int j = 0; j < HID; ++j) {
+    C v = 0;
+#pragma unroll
+    for (int i = 0; i < HID; ++i) v += P[P_W2 + i * HID + j] * z2[i];
+    const C gz1 = v * dsilu(z1[j]);
+    if (rec) rec[(R_DZ1 + j) * rstride] = gz1;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gx[k] += P[P_W1 + j * 6 + k] * gz1;
+  }
+  if (rec) {
+#pragma unroll
+    for (int i = 0; i < HID; ++i) rec[(R_DZ2 + i) * rstride] = z2[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rec[(R_DO + k) * rstride] = go[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// D3Q27 raw moments.  t[a][b][c] is the moment x^a y^b z^c (a, b, c <= 2);
+// population q sits at t[e_x + 1][e_y + 1][e_z + 1] before the transform.
+// The basis is the tensor product of the 1D moments (1, x, x^2) at
+// x in {-1, 0, 1}, so M, M^-1 and their transposes are one 3-point map
+// along each of the three axes.
+// ---------------------------------------------------------------------------
+template <int AXIS, class C, class F>
+__device__ __forceinline__ void map_axis(C (&t)[3][3][3], const F& A) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if constexpr (AXIS == 0) A(t[0][i][j], t[1][i][j], t[2][i][j]);
+      else if constexpr (AXIS == 1) A(t[i][0][j], t[i][1][j], t[i][2][j]);
+      else A(t[i][j][0], t[i][j][1], t[i][j][2]);
+    }
+}
+template <class C, class F>
+__device__ __forceinline__ void map_all(C (&t)[3][3][3], const F& A) {
+  map_axis<0>(t, A);
+  map_axis<1>(t, A);
+  map_axis<2>(t, A);
+}
+
+// t <- M v
+template <class C>
+__device__ __forceinline__ void m27_moments(const C (&v)[27], C (&t)[3][3][3]) {
+  using S = D3Q27;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1] = v[q];
+  map_all(t, [](C& a, C& b, C& c) {  // (v-, v0, v+) -> (v- + v0 + v+, v+ - v-, v+ + v-)
+    const C m0 = a + b + c, m1 = c - a, m2 = c + a;
+    a = m0, b = m1, c = m2;
+  });
+}
+
+// v <- v + sign * (M^-1 t); t is overwritten
+template <class C>
+__device__ __forceinline__ void m27_add_inverse(C (&t)[3][3][3], C sign, C (&v)[27]) {
+  using S = D3Q27;
+  map_all(t, [](C& a, C& b, C& c) {  // (m0, m1, m2) -> ((m2 - m1)/2, m0 - m2, (m1 + m2)/2)
+    const C vm = C(0.5) * (c - b), v0 = a - c, vp = C(0.5) * (b + c);
+    a = vm, b = v0, c = vp;
+  });
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] += sign * t[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];
+}
+
+// nu <- (M^-1)^T lam
+template <class C>
+__device__ __forceinline__ void m27_inverse_transposed(const C (&lam)[27], C (&nu)[3][3][3]) {
+  using S = D3Q27;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) nu[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1] = lam[q];
+  map_all(nu, [](C& a, C& b, C& c) {  // (l-, l0, l+) -> (l0, (l+ - l-)/2, (l- + l+)/2 - l0)
+    const C n0 = b, n1 = C(0.5) * (c - a), n2 = C(0.5) * (a + c) - b;
+    a = n0, b = n1, c = n2;
+  });
+}
+
+// v <- v + M^T g; g is overwritten
+template <class C>
+__device__ __forceinline__ void m27_add_transposed(C (&g)[3][3][3], C (&v)[27]) {
+  using S = D3Q27;
+  map_all(g, [](C& a, C& b, C& c) {  // (g0, g1, g2) -> (g0 - g1 + g2, g0, g0 + g1 + g2)
+    const C vm = a - b + c, v0 = a, vp = a + b + c;
+    a = vm, b = v0, c = vp;
+  });
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] += g[S::e[q][0] + 1][S::e[q][1] + 1][S::e[q][2] + 1];
+}
+
+// 1D moments sum_x W(x) x^p of W(0) = 2/3, W(+-1) = 1/6, whose products are
+// the D3Q27 weights, and their products P3(a, b, c) = raw moment abc of w
+__host__ __device__ constexpr double mu1d(int p) {
+  return p == 0 ? 1.0 : (p % 2 ? 0.0 : 1.0 / 3.0);
+}
+__host__ __device__ constexpr double p3(int a, int b, int c) { return mu1d(a) * mu1d(b) * mu1d(c); }
+
+// raw moment abc of feq(rho, u) / rho for the quadratic equilibrium
+// feq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u), and its derivatives in u
+template <class C>
+__device__ __forceinline__ C meq_unit(int a, int b, int c, const C (&u)[3], C (&du)[3]) {
+  const C uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const int ex[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  C m = (C)p3(a, b, c) * (C(1) - C(1.5) * uu);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const C pd = (C)p3(a + ex[d][0], b + ex[d][1], c + ex[d][2]);
+    m += C(3) * u[d] * pd;
+    du[d] = C(-3) * u[d] * (C)p3(a, b, c) + C(3) * pd;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const C pde = (C)p3(a + ex[d][0] + ex[e][0], b + ex[d][1] + ex[e][1], c + ex[d][2] + ex[e][2]);
+      m += C(4.5) * u[d] * u[e] * pde;
+      du[d] += C(9) * u[e] * pde;
+    }
+  }
+  return m;
+}
+
+// relaxation weight of moment abc: 1/tau on order 2, r_0 / r_1 / r_2 on
+// orders 3 / 4 / 5-6, 0 on orders 0 and 1; g = the rate group (-1: none)
+__device__ __forceinline__ int m27_group(int a, int b, int c) {
+  const int o = a + b + c;
+  return o <= 2 ? -1 : (o == 3 ? 0 : (o == 4 ? 1 : 2));
+}
+
+template <class C>
+struct CellFwd27 {
+  C rho, ri, u[3], t[3][3][3], x[6], z1[HID], z2[HID], raw[3], r[3];
+};
+
+// One D3Q27 cell's collision of the pulled populations g into out
+template <class C>
+__device__ __forceinline__ void cell_forward27(const C (&g)[27], const C* __restrict__ P,
+                                               C tau_inv, C s, C lo, C hi, CellFwd27<C>& F,
+                                               C (&out)[27], C* rec, int64_t rstride) {
+  m27_moments(g, F.t);
+  F.rho = F.t[0][0][0];
+  F.ri = C(1) / F.rho;
+  F.u[0] = F.t[1][0][0] * F.ri;
+  F.u[1] = F.t[0][1][0] * F.ri;
+  F.u[2] = F.t[0][0][1] * F.ri;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (a + b + c <= 1) continue;
+        C du[3];
+        F.t[a][b][c] -= F.rho * meq_unit(a, b, c, F.u, du);
+      }
+  const C xx = F.t[2][0][0], yy = F.t[0][2][0], zz = F.t[0][0][2];
+  const C sc = s * F.ri;
+  F.x[0] = (xx + yy + zz) * sc;
+  F.x[1] = (xx - zz) * sc;
+  F.x[2] = (yy - zz) * sc;
+  F.x[3] = F.t[1][1][0] * sc;
+  F.x[4] = F.t[1][0][1] * sc;
+  F.x[5] = F.t[0][1][1] * sc;
+  mlp_forward<C>(P, F.x, tau_inv, lo, hi, F.z1, F.z2, F.raw, F.r, rec, rstride);
+  C y[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int o = a + b + c, grp = m27_group(a, b, c);
+        y[a][b][c] = o <= 1 ? C(0) : (grp < 0 ? tau_inv : F.r[grp]) * F.t[a][b][c];
+      }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) out[q] = g[q];
+  m27_add_inverse(y, C(-1), out);
+}
+
+// mu = (d out / d g)^T lam for one D3Q27 cell; with rec set, also the record
+template <class C>
+__device__ __forceinline__ void cell_backward27(const C (&g)[27], const C (&lam)[27],
+                                                const C* __restrict__ P, C tau_inv, C s, C lo,
+                                                C hi, C (&mu)[27], C* rec, int64_t rstride) {
+  CellFwd27<C> F;
+  {
+    C fout[27];
+    cell_forward27<C>(g, P, tau_inv, s, lo, hi, F, fout, rec, rstride);
+  }
+  // out = g - Phi y, y = w . t: dL/dy = -Phi^T lam
+  C nu[3][3][3];
+  m27_inverse_transposed(lam, nu);
+  C gr[3] = {0, 0, 0};
+  C gt[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int o = a + b + c, grp = m27_group(a, b, c);
+        const C gy = -nu[a][b][c];
+        gt[a][b][c] = o <= 1 ? C(0) : (grp < 0 ? tau_inv : F.r[grp]) * gy;
+        if (grp >= 0) gr[grp] += F.t[a][b][c] * gy;
+      }
+  C go[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) go[k] = (F.raw[k] >= lo && F.raw[k] <= hi) ? gr[k] : C(0);
+  C gx[6];
+  mlp_backward<C>(P, go, F.z1, F.z2, gx, rec, rstride);
+  // x = s / rho (T, Nxz, Nyz, Pxy, Pxz, Pyz) of t
+  const C sc = s * F.ri;
+  C grho = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) grho -= gx[k] * F.x[k] * F.ri;
+  gt[2][0][0] += sc * (gx[0] + gx[1]);
+  gt[0][2][0] += sc * (gx[0] + gx[2]);
+  gt[0][0][2] += sc * (gx[0] - gx[1] - gx[2]);
+  gt[1][1][0] += sc * gx[3];
+  gt[1][0][1] += sc * gx[4];
+  gt[0][1][1] += sc * gx[5];
+  // t = M g - rho meq_unit(u) on orders >= 2
+  C gu[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (a + b + c <= 1) continue;
+        C du[3];
+        const C m = meq_unit(a, b, c, F.u, du);
+        grho -= gt[a][b][c] * m;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gu[d] -= gt[a][b][c] * F.rho * du[d];
+      }
+  // u = j / rho, rho = (M g)_000, j = (M g)_100, _010, _001
+  gt[0][0][0] = grho;
+  gt[1][0][0] = gu[0] * F.ri;
+  gt[0][1][0] = gu[1] * F.ri;
+  gt[0][0][1] = gu[2] * F.ri;
+  gt[0][0][0] -= (gu[0] * F.u[0] + gu[1] * F.u[1] + gu[2] * F.u[2]) * F.ri;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) mu[q] = lam[q];
+  m27_add_transposed(gt, mu);
+}
+
 // what the adjoint needs of the forward pass of one cell
 template <class C>
 struct CellFwd {
@@ -94,42 +415,7 @@ __device__ __forceinline__ void cell_forward(const C (&g)[9], const C* __restric
       if (MROW[3 + k][q]) m += MROW[3 + k][q] * (df[q] - ds[q]);
     F.mg[k] = m;
   }
-  C a[HID];
-#pragma unroll
-  for (int i = 0; i < HID; ++i) {
-    C z = P[P_B1 + i];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) z += P[P_W1 + i * 6 + k] * F.x[k];
-    F.z1[i] = z;
-    a[i] = z * sigmoid(z);
-  }
-  if (rec) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) rec[(R_X + k) * rstride] = F.x[k];
-#pragma unroll
-    for (int i = 0; i < HID; ++i) rec[(R_A1 + i) * rstride] = a[i];
-  }
-#pragma unroll
-  for (int i = 0; i < HID; ++i) {
-    C z = P[P_B2 + i];
-#pragma unroll
-    for (int j = 0; j < HID; ++j) z += P[P_W2 + i * HID + j] * a[j];
-    F.z2[i] = z;
-  }
-#pragma unroll
-  for (int i = 0; i < HID; ++i) a[i] = F.z2[i] * sigmoid(F.z2[i]);
-  if (rec) {
-#pragma unroll
-    for (int i = 0; i < HID; ++i) rec[(R_A2 + i) * rstride] = a[i];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    C o = P[P_B3 + k];
-#pragma unroll
-    for (int j = 0; j < HID; ++j) o += P[P_W3 + k * HID + j] * a[j];
-    F.raw[k] = tau_inv + o;
-    F.r[k] = F.raw[k] < lo ? lo : (F.raw[k] > hi ? hi : F.raw[k]);
-  }
+  mlp_forward<C>(P, F.x, tau_inv, lo, hi, F.z1, F.z2, F.raw, F.r, rec, rstride);
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     C upd = 0;
@@ -137,13 +423,6 @@ __device__ __forceinline__ void cell_forward(const C (&g)[9], const C* __restric
     for (int k = 0; k < 3; ++k) upd += F.r[k] * F.mg[k] * (C)PHI_D[k][q];
     out[q] = g[q] - ds[q] * tau_inv - upd;
   }
-}
-
-// silu'(z) = sigma(z) (1 + z (1 - sigma(z)))
-template <class C>
-__device__ __forceinline__ C dsilu(C z) {
-  const C sg = sigmoid(z);
-  return sg * (C(1) + z * (C(1) - sg));
 }
 
 // mu = (d out / d g)^T lam for one cell; with rec set, also the record
@@ -194,30 +473,8 @@ __device__ __forceinline__ void cell_backward(const C (&g)[9], const C (&lam)[9]
     if (ex * ey != 0) gdf[q] += (ex * ey > 0) ? gP : -gP;
   }
   // MLP backward; z2 is overwritten by dL/dz2
-#pragma unroll
-  for (int i = 0; i < HID; ++i) {
-    C v = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) v += P[P_W3 + k * HID + i] * go[k];
-    F.z2[i] = v * dsilu(F.z2[i]);
-  }
-  C gx[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < HID; ++j) {
-    C v = 0;
-#pragma unroll
-    for (int i = 0; i < HID; ++i) v += P[P_W2 + i * HID + j] * F.z2[i];
-    const C gz1 = v * dsilu(F.z1[j]);
-    if (rec) rec[(R_DZ1 + j) * rstride] = gz1;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) gx[k] += P[P_W1 + j * 6 + k] * gz1;
-  }
-  if (rec) {
-#pragma unroll
-    for (int i = 0; i < HID; ++i) rec[(R_DZ2 + i) * rstride] = F.z2[i];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) rec[(R_DO + k) * rstride] = go[k];
-  }
+  C gx[6];
+  mlp_backward<C>(P, go, F.z1, F.z2, gx, rec, rstride);
   // x_k = s / rho * M_k df
   C grho = 0;
 #pragma unroll
@@ -262,73 +519,83 @@ __device__ __forceinline__ void cell_tables(const KParams& p, int64_t c, int (&a
   axis_offsets(i2, p.n2, 1, a2);
 }
 
-template <class C>
+// the stencil's cell collision and its adjoint
+template <class S, class C>
+__device__ __forceinline__ void collide_cell(const C (&g)[S::Q], const C* __restrict__ P,
+                                             C tau_inv, C s, C lo, C hi, C (&out)[S::Q]) {
+  if constexpr (S::Q == 9) {
+    CellFwd<C> F;
+    cell_forward<C>(g, P, tau_inv, s, lo, hi, F, out, nullptr, 0);
+  } else {
+    CellFwd27<C> F;
+    cell_forward27<C>(g, P, tau_inv, s, lo, hi, F, out, nullptr, 0);
+  }
+}
+template <class S, class C>
+__device__ __forceinline__ void adjoint_cell(const C (&g)[S::Q], const C (&lam)[S::Q],
+                                             const C* __restrict__ P, C tau_inv, C s, C lo, C hi,
+                                             C (&mu)[S::Q], C* rec, int64_t rstride) {
+  if constexpr (S::Q == 9) cell_backward<C>(g, lam, P, tau_inv, s, lo, hi, mu, rec, rstride);
+  else cell_backward27<C>(g, lam, P, tau_inv, s, lo, hi, mu, rec, rstride);
+}
+
+template <class S, class C>
 __global__ void __launch_bounds__(128) train_forward_kernel(const KParams p, const TrainArgs t) {
-  using S = D2Q9;
   const C* P = (const C*)t.params;
   const C tau_inv = (C)p.tau_inv, s = (C)t.in_scale, lo = (C)t.rate_lo, hi = (C)t.rate_hi;
-  {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= p.ncells) return;
-    int a0[3], a1[3], a2[3];
-    cell_tables(p, c, a0, a1, a2);
-    const C* fin = (const C*)p.fin;
-    C g[9];
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= p.ncells) return;
+  int a0[3], a1[3], a2[3];
+  cell_tables(p, c, a0, a1, a2);
+  const C* fin = (const C*)p.fin;
+  C g[S::Q];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) g[q] = fin[q * p.sq + nb_offset<S>(q, -1, a0, a1, a2)];
-    CellFwd<C> F;
-    C out[9];
-    cell_forward<C>(g, P, tau_inv, s, lo, hi, F, out, nullptr, 0);
-    C* fout = (C*)p.fout;
-    const int cell = a0[1] + a1[1] + a2[1];
+  for (int q = 0; q < S::Q; ++q) g[q] = fin[q * p.sq + nb_offset<S>(q, -1, a0, a1, a2)];
+  C out[S::Q];
+  collide_cell<S, C>(g, P, tau_inv, s, lo, hi, out);
+  C* fout = (C*)p.fout;
+  const int cell = a0[1] + a1[1] + a2[1];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) fout[q * p.sq + cell] = out[q];
-  }
+  for (int q = 0; q < S::Q; ++q) fout[q * p.sq + cell] = out[q];
 }
 
 // lam: the adjoint of the step's output, either directly (gathered == 0) or
 // as the previous backward step's mu, read at x + e_q (gathered == 1)
-template <class C>
+template <class S, class C>
 __global__ void __launch_bounds__(128) train_backward_kernel(const KParams p, const TrainArgs t,
                                                              const C* __restrict__ lam_in,
                                                              int gathered, C* __restrict__ mu_out,
                                                              C* __restrict__ rec) {
-  using S = D2Q9;
   const C* P = (const C*)t.params;
   const C tau_inv = (C)p.tau_inv, s = (C)t.in_scale, lo = (C)t.rate_lo, hi = (C)t.rate_hi;
-  {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= p.ncells) return;
-    int a0[3], a1[3], a2[3];
-    cell_tables(p, c, a0, a1, a2);
-    const C* fin = (const C*)p.fin;
-    C g[9], lam[9], mu[9];
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= p.ncells) return;
+  int a0[3], a1[3], a2[3];
+  cell_tables(p, c, a0, a1, a2);
+  const C* fin = (const C*)p.fin;
+  C g[S::Q], lam[S::Q], mu[S::Q];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      g[q] = fin[q * p.sq + nb_offset<S>(q, -1, a0, a1, a2)];
-      lam[q] = lam_in[q * p.sq + (gathered ? nb_offset<S>(q, +1, a0, a1, a2) : a0[1] + a1[1] + a2[1])];
-    }
-    cell_backward<C>(g, lam, P, tau_inv, s, lo, hi, mu, rec ? rec + c : nullptr, p.ncells);
-    const int cell = a0[1] + a1[1] + a2[1];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) mu_out[q * p.sq + cell] = mu[q];
+  for (int q = 0; q < S::Q; ++q) {
+    g[q] = fin[q * p.sq + nb_offset<S>(q, -1, a0, a1, a2)];
+    lam[q] = lam_in[q * p.sq + (gathered ? nb_offset<S>(q, +1, a0, a1, a2) : a0[1] + a1[1] + a2[1])];
   }
+  adjoint_cell<S, C>(g, lam, P, tau_inv, s, lo, hi, mu, rec ? rec + c : nullptr, p.ncells);
+  const int cell = a0[1] + a1[1] + a2[1];
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) mu_out[q * p.sq + cell] = mu[q];
 }
 
 // lam_q(x) = mu_q(x + e_q): the transpose of the pull applied to mu
-template <class C>
+template <class S, class C>
 __global__ void __launch_bounds__(256) unpull_kernel(const KParams p, const C* __restrict__ mu,
                                                      C* __restrict__ lam) {
-  using S = D2Q9;
-  {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= p.ncells) return;
-    int a0[3], a1[3], a2[3];
-    cell_tables(p, c, a0, a1, a2);
-    const int cell = a0[1] + a1[1] + a2[1];
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= p.ncells) return;
+  int a0[3], a1[3], a2[3];
+  cell_tables(p, c, a0, a1, a2);
+  const int cell = a0[1] + a1[1] + a2[1];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) lam[q * p.sq + cell] = mu[q * p.sq + nb_offset<S>(q, +1, a0, a1, a2)];
-  }
+  for (int q = 0; q < S::Q; ++q) lam[q * p.sq + cell] = mu[q * p.sq + nb_offset<S>(q, +1, a0, a1, a2)];
 }
 
 }  // namespace lbm
@@ -339,20 +606,24 @@ static_assert(NPARAM == LBM_LEARNED_PARAMS, "parameter count");
 static_assert(NREC == LBM_LEARNED_RECORD, "record size");
 
 namespace {
-// periodic D2Q9 with the batch on i0, contiguous [9][n0][n1][n2]
-int train_params(int32_t storage, const int64_t n[3], double tau, KParams& p) {
+// D2Q9: periodic fields with the batch on i0, contiguous [9][n0][n1][n2];
+// D3Q27: one periodic field, contiguous [27][n0][n1][n2]
+int train_params(int32_t stencil, int32_t storage, const int64_t n[3], double tau, KParams& p) {
+  if (stencil != LBM_D2Q9 && stencil != LBM_D3Q27)
+    return report_error(LBM_E_UNSUPPORTED, "training kernels exist for D2Q9 and D3Q27");
   if (storage != LBM_F32 && storage != LBM_F64)
     return report_error(LBM_E_UNSUPPORTED, "training kernels take F32 or F64 populations");
   if (!n || n[0] < 1 || n[1] < 1 || n[2] < 1) return report_error(LBM_E_ARG, "extents must be positive");
   if (!(tau > 0.5)) return report_error(LBM_E_ARG, "tau must exceed 1/2");
   const int64_t cells = n[0] * n[1] * n[2];
-  if (9 * cells > (int64_t)0x7fffffff) return report_error(LBM_E_ARG, "more than 2^31 populations");
+  const int q = stencil == LBM_D2Q9 ? 9 : 27;
+  if (q * cells > (int64_t)0x7fffffff) return report_error(LBM_E_ARG, "more than 2^31 populations");
   memset(&p, 0, sizeof(p));
   p.n0 = (int)n[0];
   p.n1 = (int)n[1];
   p.n2 = (int)n[2];
   p.s1 = p.n2;
-  p.s0 = p.n1 * p.n2;
+  p.s0 = (long long)p.n1 * p.n2;
   p.sq = cells;
   p.ncells = cells;
   p.tau_inv = 1.0 / tau;
@@ -366,15 +637,45 @@ int launched() {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? LBM_OK : report_error(LBM_E_HIP, hipGetErrorString(e));
 }
+
+template <class C>
+void launch_forward(int32_t stencil, const KParams& p, const TrainArgs& t, hipStream_t s) {
+  const unsigned nb = train_blocks(p.ncells, 128);
+  if (stencil == LBM_D2Q9)
+    hipLaunchKernelGGL((train_forward_kernel<D2Q9, C>), dim3(nb), dim3(128), 0, s, p, t);
+  else
+    hipLaunchKernelGGL((train_forward_kernel<D3Q27, C>), dim3(nb), dim3(128), 0, s, p, t);
+}
+
+template <class C>
+void launch_backward(int32_t stencil, const KParams& p, const TrainArgs& t, const void* lam,
+                     int gathered, void* mu, void* rec, hipStream_t s) {
+  const unsigned nb = train_blocks(p.ncells, 128);
+  if (stencil == LBM_D2Q9)
+    hipLaunchKernelGGL((train_backward_kernel<D2Q9, C>), dim3(nb), dim3(128), 0, s, p, t,
+                       (const C*)lam, gathered, (C*)mu, (C*)rec);
+  else
+    hipLaunchKernelGGL((train_backward_kernel<D3Q27, C>), dim3(nb), dim3(128), 0, s, p, t,
+                       (const C*)lam, gathered, (C*)mu, (C*)rec);
+}
+
+template <class C>
+void launch_unpull(int32_t stencil, const KParams& p, const void* mu, void* lam, hipStream_t s) {
+  const unsigned nb = train_blocks(p.ncells, 256);
+  if (stencil == LBM_D2Q9)
+    hipLaunchKernelGGL((unpull_kernel<D2Q9, C>), dim3(nb), dim3(256), 0, s, p, (const C*)mu, (C*)lam);
+  else
+    hipLaunchKernelGGL((unpull_kernel<D3Q27, C>), dim3(nb), dim3(256), 0, s, p, (const C*)mu, (C*)lam);
+}
 }  // namespace
 
 extern "C" {
 
-int lbm_learned_train_forward(int32_t storage, const int64_t n[3], double tau,
+int lbm_learned_train_forward(int32_t stencil, int32_t storage, const int64_t n[3], double tau,
                               const void* params, double in_scale, double rate_lo,
                               double rate_hi, const void* f_in, void* f_out, void* stream) {
   KParams p;
-  const int rc = train_params(storage, n, tau, p);
+  const int rc = train_params(stencil, storage, n, tau, p);
   if (rc) return rc;
   if (!params || !f_in || !f_out) return report_error(LBM_E_ARG, "null buffer");
   if (f_in == f_out) return report_error(LBM_E_ARG, "f_in and f_out must differ");
@@ -382,49 +683,38 @@ int lbm_learned_train_forward(int32_t storage, const int64_t n[3], double tau,
   p.fin = f_in;
   p.fout = f_out;
   const TrainArgs t{params, in_scale, rate_lo, rate_hi};
-  const unsigned nb = train_blocks(p.ncells, 128);
-  if (storage == LBM_F64)
-    hipLaunchKernelGGL(train_forward_kernel<double>, dim3(nb), dim3(128), 0, (hipStream_t)stream, p, t);
-  else
-    hipLaunchKernelGGL(train_forward_kernel<float>, dim3(nb), dim3(128), 0, (hipStream_t)stream, p, t);
+  if (storage == LBM_F64) launch_forward<double>(stencil, p, t, (hipStream_t)stream);
+  else launch_forward<float>(stencil, p, t, (hipStream_t)stream);
   return launched();
 }
 
-int lbm_learned_train_backward(int32_t storage, const int64_t n[3], double tau,
+int lbm_learned_train_backward(int32_t stencil, int32_t storage, const int64_t n[3], double tau,
                                const void* params, double in_scale, double rate_lo,
                                double rate_hi, const void* f_in, const void* lam,
                                int32_t lam_gathered, void* mu, void* record, void* stream) {
   KParams p;
-  const int rc = train_params(storage, n, tau, p);
+  const int rc = train_params(stencil, storage, n, tau, p);
   if (rc) return rc;
   if (!params || !f_in || !lam || !mu) return report_error(LBM_E_ARG, "null buffer");
   if (lam == mu || f_in == mu) return report_error(LBM_E_ARG, "mu must not alias f_in or lam");
   if (!(rate_lo <= rate_hi)) return report_error(LBM_E_ARG, "rate_lo must not exceed rate_hi");
   p.fin = f_in;
   const TrainArgs t{params, in_scale, rate_lo, rate_hi};
-  const unsigned nb = train_blocks(p.ncells, 128);
-  if (storage == LBM_F64)
-    hipLaunchKernelGGL(train_backward_kernel<double>, dim3(nb), dim3(128), 0, (hipStream_t)stream, p,
-                       t, (const double*)lam, (int)(lam_gathered != 0), (double*)mu, (double*)record);
-  else
-    hipLaunchKernelGGL(train_backward_kernel<float>, dim3(nb), dim3(128), 0, (hipStream_t)stream, p,
-                       t, (const float*)lam, (int)(lam_gathered != 0), (float*)mu, (float*)record);
+  const int g = lam_gathered != 0;
+  if (storage == LBM_F64) launch_backward<double>(stencil, p, t, lam, g, mu, record, (hipStream_t)stream);
+  else launch_backward<float>(stencil, p, t, lam, g, mu, record, (hipStream_t)stream);
   return launched();
 }
 
-int lbm_unpull(int32_t storage, const int64_t n[3], const void* mu, void* lam, void* stream) {
+int lbm_unpull(int32_t stencil, int32_t storage, const int64_t n[3], const void* mu, void* lam,
+               void* stream) {
   KParams p;
-  const int rc = train_params(storage, n, 1.0, p);
+  const int rc = train_params(stencil, storage, n, 1.0, p);
   if (rc) return rc;
   if (!mu || !lam) return report_error(LBM_E_ARG, "null buffer");
   if (mu == lam) return report_error(LBM_E_ARG, "mu and lam must differ");
-  const unsigned nb = train_blocks(p.ncells, 256);
-  if (storage == LBM_F64)
-    hipLaunchKernelGGL(unpull_kernel<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, p,
-                       (const double*)mu, (double*)lam);
-  else
-    hipLaunchKernelGGL(unpull_kernel<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, p,
-                       (const float*)mu, (float*)lam);
+  if (storage == LBM_F64) launch_unpull<double>(stencil, p, mu, lam, (hipStream_t)stream);
+  else launch_unpull<float>(stencil, p, mu, lam, (hipStream_t)stream);
   return launched();
 }
 

@@ -8,6 +8,7 @@ f - feq into a shear part and a higher-order part
 formula are described in csrc/lbm_learned.hip and include/lettuce_amd.h.
 """
 import ctypes
+import os
 from typing import Optional
 
 import numpy as np
@@ -18,7 +19,13 @@ from .._simulation import Collision
 from .._stencil import D2Q9, D3Q27
 from ._operators import NativeCollisionDesc, apply_collision
 
-__all__ = ["LearnedParameters", "LearnedCollision"]
+__all__ = ["LearnedParameters", "LearnedCollision", "TRAINED_D3Q27_KBC"]
+
+# D3Q27 weights trained on decaying-turbulence states of KBC runs
+# (python -m lettuce_amd.studies train3d --out ...): 64^3, u_rms 0.05,
+# nu 1e-4, tau 0.5003, 16-step rollouts against KBC's velocity
+TRAINED_D3Q27_KBC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                 "data", "learned_d3q27_kbc.npz")
 
 
 class LearnedParameters:
@@ -63,6 +70,26 @@ class LearnedParameters:
 
     def tensors(self):
         return [self.w1, self.b1, self.w2, self.b2, self.w3, self.b3]
+
+    _LIMITS = ("in_scale", "rate_lo", "rate_hi", "stable_lo", "stable_hi")
+
+    def save(self, path):
+        """weights (FP32) and limits to an .npz file"""
+        arrays = {k: t.numpy() for k, t in zip(("w1", "b1", "w2", "b2", "w3", "b3"),
+                                               self.tensors())}
+        arrays.update({k: np.array(getattr(self, k), dtype=np.float64) for k in self._LIMITS})
+        with open(path, "wb") as fh:
+            np.savez(fh, **arrays)
+
+    @staticmethod
+    def load(path, **overrides) -> "LearnedParameters":
+        """parameters written by save(); keyword arguments replace limits"""
+        with np.load(path, allow_pickle=False) as z:
+            w = [torch.from_numpy(z[k].astype(np.float32)) for k in
+                 ("w1", "b1", "w2", "b2", "w3", "b3")]
+            kw = {k: float(z[k]) for k in LearnedParameters._LIMITS}
+        kw.update(overrides)
+        return LearnedParameters(*w, **kw)
 
     def packed(self, device) -> torch.Tensor:
         """The MFMA operand buffer of lbm_learned_pack, on `device`."""

@@ -5,8 +5,9 @@ learned collision of ._learned (the MLP 6 -> 32 -> 32 -> 3 that sets the
 relaxation rates of the moments xxy, xyy, xxyy on top of KBC's shear split,
 lettuce/ext/_collision/kbc_collision.py:76-160).
 
-LearnedRollout is a torch.autograd.Function over `steps` periodic D2Q9 pull
-steps f_{n+1} = C(S f_n) of a batch of fields [9, B, X, Y].  Its backward
+LearnedRollout is a torch.autograd.Function over `steps` periodic pull
+steps f_{n+1} = C(S f_n) of a batch of D2Q9 fields [9, B, X, Y] or of one
+D3Q27 field [27, X, Y, Z].  Its backward
 does not record a graph; it runs the adjoint kernels of csrc/lbm_train.hip:
 
   mu_n          = (dC/dg)^T lambda_{n+1}     per cell (lbm_learned_train_backward)
@@ -82,20 +83,26 @@ _STORAGE = {torch.float32: N.F32, torch.float64: N.F64}
 
 
 class _Problem:
-    """extents, storage code and stream of one rollout"""
+    """stencil, extents, storage code and stream of one rollout"""
 
     def __init__(self, f: torch.Tensor, theta: torch.Tensor, cfg: TrainConfig):
         N.require_device_tensor(f, "f")
         if f.dtype not in _STORAGE:
             raise RuntimeError("training runs on float32 or float64 populations")
-        if f.dim() not in (3, 4) or f.shape[0] != 9:
-            raise RuntimeError("populations must be [9, X, Y] or [9, B, X, Y] (D2Q9)")
+        if f.shape[0] == 9 and f.dim() in (3, 4):
+            self.stencil = N.D2Q9
+            shape = f.shape[1:] if f.dim() == 4 else (1,) + tuple(f.shape[1:])
+        elif f.shape[0] == 27 and f.dim() == 4:
+            self.stencil = N.D3Q27
+            shape = tuple(f.shape[1:])
+        else:
+            raise RuntimeError("populations must be [9, X, Y] or [9, B, X, Y] (D2Q9) "
+                               "or [27, X, Y, Z] (D3Q27)")
         if not f.is_contiguous():
             raise RuntimeError("populations must be contiguous")
         if theta.dtype != f.dtype or theta.device != f.device or theta.numel() != N.LEARNED_PARAMS:
             raise RuntimeError(f"theta must hold {N.LEARNED_PARAMS} values of the populations' "
                                "dtype on their device")
-        shape = f.shape[1:] if f.dim() == 4 else (1,) + tuple(f.shape[1:])
         self.n = (ctypes.c_int64 * 3)(*[int(v) for v in shape])
         self.cells = int(shape[0] * shape[1] * shape[2])
         self.storage = _STORAGE[f.dtype]
@@ -106,20 +113,20 @@ class _Problem:
     def step(self, src, dst):
         c = self.cfg
         N.check(N.lib().lbm_learned_train_forward(
-            self.storage, self.n, c.tau, self.theta.data_ptr(), c.in_scale, c.rate_lo,
+            self.stencil, self.storage, self.n, c.tau, self.theta.data_ptr(), c.in_scale, c.rate_lo,
             c.rate_hi, src.data_ptr(), dst.data_ptr(), self.stream),
             "lbm_learned_train_forward")
 
     def adjoint(self, f, lam, gathered, mu, record):
         c = self.cfg
         N.check(N.lib().lbm_learned_train_backward(
-            self.storage, self.n, c.tau, self.theta.data_ptr(), c.in_scale, c.rate_lo,
+            self.stencil, self.storage, self.n, c.tau, self.theta.data_ptr(), c.in_scale, c.rate_lo,
             c.rate_hi, f.data_ptr(), lam.data_ptr(), int(gathered), mu.data_ptr(),
             record.data_ptr() if record is not None else None, self.stream),
             "lbm_learned_train_backward")
 
     def unpull(self, mu, lam):
-        N.check(N.lib().lbm_unpull(self.storage, self.n, mu.data_ptr(), lam.data_ptr(),
+        N.check(N.lib().lbm_unpull(self.stencil, self.storage, self.n, mu.data_ptr(), lam.data_ptr(),
                                    self.stream), "lbm_unpull")
 
 
@@ -187,8 +194,8 @@ class LearnedRollout(torch.autograd.Function):
 
 
 def learned_rollout(f0, theta, cfg: TrainConfig, steps=16, checkpoint_every=4):
-    """`steps` learned-collision steps of f0 [9, (B,) X, Y] with the hand-written
-    adjoint as backward"""
+    """`steps` learned-collision steps of f0 [9, (B,) X, Y] or [27, X, Y, Z]
+    with the hand-written adjoint as backward"""
     return LearnedRollout.apply(f0, theta, cfg, steps, checkpoint_every)
 
 

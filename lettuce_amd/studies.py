@@ -6,6 +6,7 @@ same case on the GPU.  Prints one JSON object per study.
     python -m lettuce_amd.studies cylinder     # D2Q9 channel + cylinder, KBC vs learned
     python -m lettuce_amd.studies turbulence3d # D3Q27 KBC decaying turbulence 384^3
     python -m lettuce_amd.studies train2d      # training of the learned collision
+    python -m lettuce_amd.studies train3d      # D3Q27 training on KBC states
     python -m lettuce_amd.studies slab         # one device's x-slab vs the periodic step
     python -m lettuce_amd.studies all
 
@@ -373,6 +374,167 @@ def train2d(iters=300, n=256, fields=32, holdout=8, steps=16, every=4, u_rms=0.0
             "wall_s": round(time.perf_counter() - t_all, 2)}
 
 
+def turbulence_units(res, u_rms=0.05, nu=1e-4, k0=4, seed=0):
+    """(Re, Ma) that give lettuce's DecayingTurbulence (D3Q27, res^3, seed)
+    the lattice-unit rms velocity u_rms and viscosity nu: lettuce scales the
+    field to max |u| = Ma cs (decayingturbulence.py:167-169), so Ma follows
+    from the field's rms / max ratio"""
+    probe = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float64), [res] * 3, 1.0,
+                                  0.1, k0=k0, stencil=lt.D3Q27(), initialize_pressure=False,
+                                  initialize_fneq=False, randseed=seed)
+    u = probe.u()
+    ratio = float(torch.sqrt((u * u).sum(0).mean()) / torch.linalg.vector_norm(u, dim=0).max())
+    del probe, u
+    torch.cuda.empty_cache()
+    u_max = u_rms / ratio
+    return res * u_max / nu, u_max / lt.D3Q27().cs
+
+
+class _StateFlow(lt.ExtFlow):
+    """periodic flow that starts from given populations f0 [Q, *res] and
+    uses a given UnitConversion"""
+
+    def __init__(self, context, f0, units, stencil):
+        self._f0, self._units = f0, units
+        super().__init__(context, list(f0.shape[1:]), 1.0, 1.0, stencil)
+
+    def make_resolution(self, resolution, stencil=None):
+        return list(resolution)
+
+    def make_units(self, reynolds_number, mach_number, resolution):
+        return self._units
+
+    def initial_pu(self):
+        raise NotImplementedError
+
+    def initialize(self):
+        self.f = self._f0
+
+
+def train3d(iters=150, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=0,
+            starts=(0, 100, 200, 300), lr=2e-3, eval_every=25, rate_hi=1.999, out=None):
+    """Training of the D3Q27 learned collision on states that KBC generates.
+
+    lettuce's DecayingTurbulence (D3Q27, E(k) ~ k^4 exp(-2 (k/k0)^2), u_rms
+    and nu in lattice units as in the 384^3 benchmark, hence the same tau)
+    at n^3 runs with KBC (FP32, the production kernel).  Its states after
+    `starts` steps are the rollouts' starts and its states `steps` steps
+    later the targets.  Model: `steps` steps of the learned collision's
+    training formula from the same starts.  Loss: mean |u - u_KBC|^2 over the
+    cells, divided by the same for BGK (a zero MLP), averaged over the
+    starts.  Adam, the hand-written D3Q27 adjoint with checkpoints every
+    `every` steps.  Held out: the same from the field of seed + 1.  The
+    trained weights then run in the bf16 MFMA inference kernel on the
+    held-out starts: loss and the fraction of cells that fall back to KBC.
+    With `out`, the trained parameters are saved there (LearnedParameters.save)."""
+    dev = torch.device("cuda:0")
+    t_all = time.perf_counter()
+    re, ma = turbulence_units(n, u_rms, nu, k0, seed)
+
+    def kbc_states(field_seed):
+        flow = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float32), [n] * 3, re,
+                                     ma, k0=k0, stencil=lt.D3Q27(), initialize_pressure=False,
+                                     initialize_fneq=False, randseed=field_seed)
+        sim = lt.Simulation(flow, lt.KBCCollision(), [], lt.StreamingStrategy.PRE_STREAMING)
+        pairs, t = [], 0
+        for s0 in starts:
+            sim(s0 - t)
+            f0 = flow.f.contiguous().clone()
+            sim(steps)
+            t = s0 + steps
+            pairs.append((f0, flow.u().float().clone()))
+        return flow.units, pairs
+
+    units, train = kbc_states(seed)
+    _, test = kbc_states(seed + 1)
+    tau = units.relaxation_parameter_lu
+    zero = torch.zeros(N.LEARNED_PARAMS, dtype=torch.float32, device=dev)
+    e = torch.tensor(lt.D3Q27().e, dtype=torch.float32, device=dev)
+
+    def vel(f):
+        return torch.einsum("qd,q...->d...", e, f) / f.sum(0)
+
+    def err(u, ut):
+        return ((u - ut) ** 2).sum(0).mean()
+
+    base = lt.LearnedParameters.seeded(seed, out_scale=0.1, rate_hi=rate_hi,
+                                       stable_lo=-1e30, stable_hi=1e30)
+    cfg = lt.TrainConfig.of(base, tau)
+    with torch.no_grad():
+        bgk_tr = [err(vel(lt.learned_rollout(f0, zero, cfg, steps, every)), ut)
+                  for f0, ut in train]
+        bgk_te = [err(vel(lt.learned_rollout(f0, zero, cfg, steps, every)), ut)
+                  for f0, ut in test]
+    theta = lt.flatten_parameters(base, torch.float32, dev).requires_grad_(True)
+    opt = torch.optim.Adam([theta], lr=lr)
+    history = []
+
+    def test_loss(th):
+        with torch.no_grad():
+            return float(sum(err(vel(lt.learned_rollout(f0, th, cfg, steps, every)), ut) / b
+                             for (f0, ut), b in zip(test, bgk_te)) / len(test))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    first = None
+    for it in range(iters + 1):
+        opt.zero_grad()
+        loss = sum(err(vel(lt.learned_rollout(f0, theta, cfg, steps, every)), ut) / b
+                   for (f0, ut), b in zip(train, bgk_tr)) / len(train)
+        if first is None:
+            first = float(loss)
+        if it % eval_every == 0 or it == iters:
+            history.append({"iter": it, "train": float(loss), "test": test_loss(theta)})
+            print(f"train3d iter {it}: train {history[-1]['train']:.4f} "
+                  f"test {history[-1]['test']:.4f}", file=sys.stderr, flush=True)
+        if it == iters:
+            break
+        loss.backward()
+        opt.step()
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # the production limits: rates clamped below 2, the KBC fallback outside [0, 2]
+    trained = lt.unflatten_parameters(theta, like=base)
+    trained.stable_lo, trained.stable_hi = 0.0, 2.0
+    if out:
+        trained.save(out)
+
+    def production(make_collision):
+        total, fallback = 0.0, 0.0
+        for (f0, ut), b in zip(test, bgk_te):
+            flow = _StateFlow(lt.Context("cuda:0", dtype=torch.float32), f0.clone(), units,
+                              lt.D3Q27())
+            coll = make_collision()
+            if isinstance(coll, lt.LearnedCollision):
+                p = coll.params
+                free = lt.LearnedParameters(*p.tensors(), in_scale=p.in_scale,
+                                            rate_lo=p.rate_lo, rate_hi=p.rate_hi,
+                                            stable_lo=-1e30, stable_hi=1e30)
+                a = lt.LearnedCollision(p)(flow)
+                b_ = lt.LearnedCollision(free)(flow)
+                fallback += float((a != b_).any(0).float().mean())
+            sim = lt.Simulation(flow, coll, [], lt.StreamingStrategy.PRE_STREAMING)
+            sim(steps)
+            total += float(err(flow.u().float(), ut) / b)
+        return total / len(test), fallback / len(test)
+
+    prod = {"kbc": production(lambda: lt.KBCCollision())[0]}
+    for name, params in (("learned_initial", lt.LearnedParameters.seeded(seed, out_scale=0.1)),
+                         ("learned_trained", trained)):
+        loss_p, fb = production(lambda: lt.LearnedCollision(params))
+        prod[name] = loss_p
+        prod[name + "_fallback_cell_frac"] = fb
+    return {"study": "train3d_d3q27_learned_kbc_targets", "grid": [n] * 3, "starts": list(starts),
+            "steps": steps, "checkpoint_every": every, "tau": tau, "u_rms": u_rms, "nu": nu,
+            "k0": k0, "iters": iters, "lr": lr, "rate_hi": rate_hi,
+            "loss_initial": first, "loss_final_train": history[-1]["train"],
+            "loss_final_test": history[-1]["test"],
+            "note": "losses are mean |u - u_KBC|^2 after `steps` steps divided by BGK's",
+            "production_kernels_test_loss": prod, "history": history,
+            "train_s": round(t_train, 2), "ms_per_iter": round(1e3 * t_train / max(iters, 1), 2),
+            "weights": out, "wall_s": round(time.perf_counter() - t_all, 2)}
+
+
 def slab(steps=100, stencil="D3Q27", collision="kbc", res=384, planes=48, storage="f32"):
     """One device's share of a res^3 lattice split into x-slabs of `planes`
     planes (48 x 384 x 384: 384^3 over 8 devices), initialised from the
@@ -449,7 +611,8 @@ def slab(steps=100, stencil="D3Q27", collision="kbc", res=384, planes=48, storag
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("study", choices=["tgv2d", "tgv3d", "cylinder", "turbulence3d", "train2d",
-                                      "slab", "all"])
+                                      "train3d", "slab", "all"])
+    ap.add_argument("--out", default=None, help="train3d: file for the trained parameters")
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--res", type=int, default=None)
@@ -470,6 +633,10 @@ def main():
     if args.study in ("train2d", "all"):
         kw2 = dict(kw, **({"iters": args.iters} if args.iters else {}))
         print(json.dumps(train2d(**kw2, **({"n": args.res} if args.res else {}))), flush=True)
+    if args.study in ("train3d", "all"):
+        kw3 = dict(kw, **({"iters": args.iters} if args.iters else {}))
+        print(json.dumps(train3d(**kw3, out=args.out, **({"n": args.res} if args.res else {}))),
+              flush=True)
     if args.study in ("turbulence3d", "all"):
         print(json.dumps(turbulence3d(**kw, **({"res": args.res} if args.res else {}))),
               flush=True)

@@ -1109,3 +1109,50 @@ def learned_train_step(f, theta, tau, **kw):
     g = torch.stack([torch.roll(f[q], shifts=tuple(D2Q9.e[q]), dims=(-2, -1))
                      for q in range(9)])
     return learned_train_collide(g, theta, tau, **kw)
+
+
+def learned_train_collide_d3q27(f, theta, tau, in_scale=100.0, rate_lo=0.05, rate_hi=1.95):
+    """D3Q27 LearnedCollision's formula (KBC's shear part ds of df = f - feq,
+    dh = df - ds, the rates acting on the raw moments of dh of order 3, 4 and
+    5-6 through the dual basis phi = M^-1) with the MLP evaluated in f's
+    dtype and without the fallback; clamp() passes the gradient inside
+    [rate_lo, rate_hi] only."""
+    dtype = f.dtype
+    e = torch.tensor(D3Q27.e, dtype=dtype)
+    w = torch.tensor(D3Q27.w, dtype=dtype)
+    rho = f.sum(0)
+    u = torch.einsum("qd,q...->d...", e, f) / rho
+    eu = torch.einsum("qd,d...->q...", e, u)
+    uu = (u * u).sum(0)
+    feq = w.reshape([-1] + [1] * u[0].dim()) * rho * (1 + 3 * eu - 1.5 * uu + 4.5 * eu * eu)
+    df = f - feq
+    xx, yy, zz, xy, xz, yz = d3q27_second_moments(df)
+    T, Nxz, Nyz = xx + yy + zz, xx - zz, yy - zz
+    a, b, c = (2 * Nxz - Nyz + T) / 6, (2 * Nyz - Nxz + T) / 6, (-Nxz - Nyz + T) / 6
+    zero = torch.zeros_like(T)
+    # kbc_collision.py:41-74 with r = 1: the shear part fills s[0..18]
+    ds = torch.stack([-T, a, a, b, b, c, c, 0.25 * yz, 0.25 * yz, -0.25 * yz, -0.25 * yz,
+                      0.25 * xz, 0.25 * xz, -0.25 * xz, -0.25 * xz, 0.25 * xy, 0.25 * xy,
+                      -0.25 * xy, -0.25 * xy] + [zero] * 8)
+    dh = df - ds
+    M, orders = d3q27_raw_moments(dtype)
+    phi = torch.linalg.inv(M)
+    m = torch.einsum("kq,q...->k...", M, dh)
+    w1, b1, w2, b2, w3, b3 = split_learned_params(theta)
+    x = torch.movedim(torch.stack([T, Nxz, Nyz, xy, xz, yz]) / rho * in_scale, 0, -1)
+    h = torch.nn.functional.silu(torch.nn.functional.linear(x, w1, b1))
+    h = torch.nn.functional.silu(torch.nn.functional.linear(h, w2, b2))
+    o = torch.movedim(torch.nn.functional.linear(h, w3, b3), -1, 0)
+    rates = (1.0 / tau + o).clamp(rate_lo, rate_hi)
+    group = [-1 if k <= 2 else (0 if k == 3 else (1 if k == 4 else 2)) for k in orders]
+    upd = torch.stack([rates[g] * m[k] if g >= 0 else torch.zeros_like(m[k])
+                       for k, g in enumerate(group)])
+    return f - ds / tau - torch.einsum("qk,k...->q...", phi, upd)
+
+
+def learned_train_step_d3q27(f, theta, tau, **kw):
+    """periodic pull streaming of f [27, X, Y, Z] (lettuce/_simulation.py:241-256),
+    then learned_train_collide_d3q27"""
+    g = torch.stack([torch.roll(f[q], shifts=tuple(D3Q27.e[q]), dims=(0, 1, 2))
+                     for q in range(27)])
+    return learned_train_collide_d3q27(g, theta, tau, **kw)

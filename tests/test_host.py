@@ -76,20 +76,26 @@ def test_training_entry_points_reject_bad_arguments_without_a_gpu():
     n = (ctypes.c_int64 * 3)(2, 8, 8)
     p, a, b = ctypes.c_void_p(4096), ctypes.c_void_p(8192), ctypes.c_void_p(16384)
     fwd = lib.lbm_learned_train_forward
-    assert fwd(N.F16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -3
-    assert fwd(N.F32, n, 0.5, p, 100.0, 0.05, 1.95, a, b, None) == -1
+    D2, D3 = N.D2Q9, N.D3Q27
+    assert fwd(D2, N.F16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -3
+    assert fwd(N.D3Q19, N.F32, n, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -3
+    assert fwd(D2, N.F32, n, 0.5, p, 100.0, 0.05, 1.95, a, b, None) == -1
     assert b"tau" in lib.lbm_last_error()
-    assert fwd(N.F32, n, 0.6, p, 100.0, 0.05, 1.95, a, a, None) == -1
-    assert fwd(N.F32, n, 0.6, None, 100.0, 0.05, 1.95, a, b, None) == -1
-    assert fwd(N.F32, n, 0.6, p, 100.0, 1.5, 1.0, a, b, None) == -1
+    assert fwd(D3, N.F32, n, 0.6, p, 100.0, 0.05, 1.95, a, a, None) == -1
+    assert fwd(D2, N.F32, n, 0.6, None, 100.0, 0.05, 1.95, a, b, None) == -1
+    assert fwd(D2, N.F32, n, 0.6, p, 100.0, 1.5, 1.0, a, b, None) == -1
     zero = (ctypes.c_int64 * 3)(2, 0, 8)
-    assert fwd(N.F64, zero, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -1
+    assert fwd(D2, N.F64, zero, 0.6, p, 100.0, 0.05, 1.95, a, b, None) == -1
     bwd = lib.lbm_learned_train_backward
-    assert bwd(N.F64, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, b, None, None) == -1
-    assert bwd(N.BF16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, p, None, None) == -3
-    assert lib.lbm_unpull(N.F32, n, a, a, None) == -1
+    assert bwd(D2, N.F64, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, b, None, None) == -1
+    assert bwd(D3, N.BF16_SHIFTED, n, 0.6, p, 100.0, 0.05, 1.95, a, b, 1, p, None, None) == -3
+    assert lib.lbm_unpull(D2, N.F32, n, a, a, None) == -1
     big = (ctypes.c_int64 * 3)(1 << 20, 1 << 10, 1 << 10)
-    assert lib.lbm_unpull(N.F32, big, a, b, None) == -1
+    assert lib.lbm_unpull(D2, N.F32, big, a, b, None) == -1
+    # 27 populations of 2^27 cells pass 2^31 (9 of them would not)
+    mid = (ctypes.c_int64 * 3)(1 << 7, 1 << 10, 1 << 10)
+    assert lib.lbm_unpull(D3, N.F32, mid, a, b, None) == -1
+    assert b"2^31" in lib.lbm_last_error()
 
 
 def test_scan_rejects_bad_arguments_without_a_gpu():

@@ -216,3 +216,68 @@ def test_d3q27_learned_conserves_mass_and_momentum():
     torch.testing.assert_close(post.sum(0), f.sum(0), rtol=0, atol=1e-14)
     torch.testing.assert_close(torch.einsum("qd,q...->d...", e, post),
                                torch.einsum("qd,q...->d...", e, f), rtol=0, atol=1e-14)
+
+
+def test_d3q27_train_collide_zero_mlp_is_bgk():
+    tau = 0.57
+    f = _grid27(seed=6)
+    out = O.learned_train_collide_d3q27(f, torch.zeros(O.LEARNED_PARAMS, dtype=F64), tau)
+    torch.testing.assert_close(out, O.BGKCollision(tau)(_Flow27(f, tau)), rtol=0, atol=1e-15)
+
+
+def test_d3q27_train_collide_constant_rates_match_inference_formula():
+    """w3 = 0, b3 = c: the training step equals the D3Q27 inference formula
+    with forced rates 1/tau + c (that formula reduces to KBC at beta gamma)"""
+    tau = 0.6
+    c = torch.tensor([0.1, -0.2, 0.25], dtype=F64)
+    theta = _theta(3)
+    theta[-3 - 3 * 32:-3] = 0
+    theta[-3:] = c
+    f = _grid27(seed=8)
+    lc = O.LearnedCollision(O.LearnedParameters.seeded(0))
+    lc.force_rates = (1 / tau + c).tolist()
+    ref = lc(_Flow27(f, tau))
+    torch.testing.assert_close(O.learned_train_collide_d3q27(f, theta, tau), ref, rtol=0,
+                               atol=1e-15)
+
+
+def test_d3q27_train_collide_conserves_mass_and_momentum():
+    f = _grid27(seed=2)
+    out = O.learned_train_collide_d3q27(f, _theta(1, out_scale=1.0), 0.53)
+    e = torch.tensor(O.D3Q27.e, dtype=F64)
+    torch.testing.assert_close(out.sum(0), f.sum(0), rtol=0, atol=1e-14)
+    torch.testing.assert_close(torch.einsum("qd,q...->d...", e, out),
+                               torch.einsum("qd,q...->d...", e, f), rtol=0, atol=1e-14)
+
+
+def test_d3q27_train_step_streams_like_the_simulation():
+    tau = 0.7
+    f = _grid27(n=3, seed=7)
+    flow = _Flow27(f, tau)
+    O.Simulation(flow, O.BGKCollision(tau), [], O.StreamingStrategy.PRE_STREAMING)(1)
+    out = O.learned_train_step_d3q27(f, torch.zeros(O.LEARNED_PARAMS, dtype=F64), tau)
+    torch.testing.assert_close(out, flow.f, rtol=0, atol=1e-15)
+
+
+def test_d3q27_train_step_autograd_matches_central_differences():
+    tau = 0.58
+    theta = _theta(5, out_scale=0.3)
+    f0 = _grid27(n=3, seed=9)
+    g = torch.Generator().manual_seed(1)
+    wgt = torch.rand(f0.shape, generator=g, dtype=F64)
+
+    def loss(f, th):
+        for _ in range(2):
+            f = O.learned_train_step_d3q27(f, th, tau)
+        return (wgt * f).sum() + 10 * (f.sum(0) ** 2).sum()
+
+    th = theta.clone().requires_grad_(True)
+    ff = f0.clone().requires_grad_(True)
+    loss(ff, th).backward()
+    for which, x, grad in (("theta", theta, th.grad), ("f", f0, ff.grad)):
+        d = torch.randn(x.shape, generator=g, dtype=F64)
+        h = 1e-6
+        num = (loss(*((f0 + h * d, theta) if which == "f" else (f0, theta + h * d)))
+               - loss(*((f0 - h * d, theta) if which == "f" else (f0, theta - h * d)))) / (2 * h)
+        ana = (grad * d).sum()
+        assert abs(float(num - ana)) <= 1e-6 * max(1.0, abs(float(ana))), (which, num, ana)

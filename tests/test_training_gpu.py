@@ -187,3 +187,93 @@ def test_training_reduces_the_loss():
         opt.step()
         losses.append(loss.item())
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+# ------------------------------------------------------------------ D3Q27
+from tests.test_learned_oracle import _grid27  # noqa: E402
+
+
+def _oracle_rollout27(f0, theta, tau, steps):
+    f = f0
+    for _ in range(steps):
+        f = O.learned_train_step_d3q27(f, theta, tau)
+    return f
+
+
+@pytest.mark.parametrize("steps", [1, 5])
+def test_d3q27_forward_matches_oracle_f64(steps):
+    """the D3Q27 training forward (raw-moment formulation) against the
+    oracle's (KBC shear split and dual basis), FP64, a 6 x 5 x 4 field"""
+    tau = 0.58
+    f0 = _grid27(n=6, seed=1)[:, :, :5, :4].contiguous()
+    theta = _theta(2, out_scale=0.5)
+    got = _rollout(f0, theta, tau, steps).cpu()
+    ref = _oracle_rollout27(f0, theta, tau, steps)
+    err = (got - ref).abs().max().item()
+    assert err <= 1e-14, err
+
+
+def test_d3q27_unpull_is_the_transpose_of_pull():
+    f = torch.rand(27, 5, 4, 6, dtype=F64, device=DEV)
+    from lettuce_amd.ext._training import _Problem
+    prob = _Problem(f, torch.zeros(1379, dtype=F64, device=DEV), _cfg(0.6))
+    lam = torch.empty_like(f)
+    prob.unpull(f, lam)
+    e = O.D3Q27.e
+    ref = torch.stack([torch.roll(f[q], shifts=tuple(-c for c in e[q]), dims=(0, 1, 2))
+                       for q in range(27)])
+    assert torch.equal(lam, ref)
+
+
+@pytest.mark.parametrize("steps,every", [(4, 4), (5, 2)])
+def test_d3q27_adjoint_matches_torch_autograd(steps, every):
+    """the D3Q27 hand-written adjoint against torch autograd through the
+    oracle's training step, FP64"""
+    tau = 0.56
+    f0 = _grid27(n=5, seed=5)[:, :, :, :4].contiguous()
+    theta = _theta(6, out_scale=0.4)
+    wgt = torch.rand(f0.shape, dtype=F64, generator=torch.Generator().manual_seed(3))
+
+    def loss(f, w):
+        e = torch.tensor(O.D3Q27.e, dtype=F64, device=f.device)
+        rho = f.sum(0)
+        u = torch.einsum("qd,q...->d...", e, f) / rho
+        return (w * f).sum() + 10.0 * (rho ** 2).sum() + 100.0 * (u ** 2).sum()
+
+    f_r = f0.clone().requires_grad_(True)
+    th_r = theta.clone().requires_grad_(True)
+    loss(_oracle_rollout27(f_r, th_r, tau, steps), wgt).backward()
+    f_g = f0.to(DEV).requires_grad_(True)
+    th_g = theta.to(DEV).requires_grad_(True)
+    loss(lt.learned_rollout(f_g, th_g, _cfg(tau), steps, every), wgt.to(DEV)).backward()
+    for name, a, b in (("theta", th_g.grad.cpu(), th_r.grad), ("f0", f_g.grad.cpu(), f_r.grad)):
+        rel = ((a - b).abs().max() / b.abs().max()).item()
+        assert rel <= 1e-10, (name, rel)
+
+
+def test_d3q27_adjoint_central_differences_16_steps():
+    """FP64, 16-step unroll with checkpoints every 4, an 8^3 field: central
+    differences with a relative step of 1e-4 agree with the adjoint to 1e-3"""
+    tau = 0.55
+    f0 = _grid27(n=8, seed=8).to(DEV)
+    theta = _theta(7, out_scale=0.5).to(DEV)
+    target = _grid27(n=8, seed=9).to(DEV)
+    e = torch.tensor(O.D3Q27.e, dtype=F64, device=DEV)
+
+    def vel(f):
+        return torch.einsum("qd,q...->d...", e, f) / f.sum(0)
+
+    u_t = vel(target)
+
+    def loss(th):
+        return ((vel(lt.learned_rollout(f0, th, _cfg(tau), 16, 4)) - u_t) ** 2).mean() * 1e4
+
+    th = theta.clone().requires_grad_(True)
+    loss(th).backward()
+    g = torch.Generator(device=DEV).manual_seed(11)
+    h = 1e-4
+    d = theta * torch.randn(theta.shape, generator=g, dtype=F64, device=DEV)
+    with torch.no_grad():
+        num = (loss(theta + h * d) - loss(theta - h * d)) / (2 * h)
+    ana = (th.grad * d).sum()
+    assert abs((num - ana) / ana).item() <= 1e-3, (num.item(), ana.item())
