@@ -335,20 +335,31 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
     after `warmup`; and the largest relative kinetic-energy error of each
     against KBC with FP64 storage, every `every` steps over `steps` steps."""
     t_all = time.perf_counter()
+    from lettuce_amd.ext._learned import TRAINED_D3Q27_KBC
+    if os.path.exists(TRAINED_D3Q27_KBC):
+        params = lt.LearnedParameters.load(TRAINED_D3Q27_KBC)
+        weights = ("trained on KBC decaying-turbulence states at 64^3 "
+                   "(lettuce_amd/data/learned_d3q27_kbc.npz)")
+    else:
+        params = lt.LearnedParameters.seeded(seed, out_scale=0.1)
+        weights = f"LearnedParameters.seeded({seed}, out_scale=0.1)"
+    free = lt.LearnedParameters(*params.tensors(), in_scale=params.in_scale,
+                                rate_lo=params.rate_lo, rate_hi=params.rate_hi,
+                                stable_lo=-1e30, stable_hi=1e30)
     field = _SharedTurbulence(lt, res, u_rms, nu, k0, seed)
     runs = [("kbc_f64", torch.float64, "f64", "kbc"), ("kbc_f32", torch.float32, "f32", "kbc"),
             ("learned_f16s", torch.float32, "f16_shifted", "learned"),
             ("learned_bf16s", torch.float32, "bf16_shifted", "learned")]
     out = {"workload": f"turb3d_d3q27_{res}", "grid": [res] * 3, "cells": res ** 3,
            "u_rms_lu": u_rms, "nu_lu": nu, "ke_steps": steps, "ke_every": every,
-           "timed_steps": timed, "reference": "kbc_f64",
-           "learned_weights": "LearnedParameters.seeded(0, out_scale=0.1)"}
+           "timed_steps": timed, "reference": "kbc_f64", "learned_weights": weights,
+           "rate_clamp": [params.rate_lo, params.rate_hi],
+           "kbc_fallback_outside": [params.stable_lo, params.stable_hi]}
     ke = {}
     for name, dtype, storage, coll in runs:
         flow = field.flow(lt, lt.Context("cuda:0", dtype=dtype, storage=storage))
         out["tau"] = flow.units.relaxation_parameter_lu
-        collision = (lt.KBCCollision() if coll == "kbc" else
-                     lt.LearnedCollision(lt.LearnedParameters.seeded(seed, out_scale=0.1)))
+        collision = lt.KBCCollision() if coll == "kbc" else lt.LearnedCollision(params)
         sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
         energy = lt.IncompressibleKineticEnergy(flow)
         series = [float(energy())]
@@ -357,6 +368,14 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
             series.append(float(energy()))
         ke[name] = np.array(series)
         entry = {"storage": storage, "collision": coll.upper()}
+        if coll == "learned":
+            # cells that took the KBC fallback in the last state's collision:
+            # where the learned collision differs from the same one with an
+            # unbounded stable range
+            a = lt.LearnedCollision(params)(flow)
+            b = lt.LearnedCollision(free)(flow)
+            entry["kbc_fallback_cell_frac"] = float((a != b).any(0).float().mean())
+            del a, b
         if name != "kbc_f64":
             gpu_s, _ = time_gpu(sim, timed, warmup)
             entry["mlups"] = round(timed * res ** 3 / 1e6 / gpu_s, 1)

@@ -42,3 +42,19 @@ def knobs():
     yield set_knobs
     for name in touched:
         N.lib().lbm_set_tuning(name.encode(), -1)
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """with LBM_ERR_LOG set, the max-normalised and entrywise errors of every
+    rel_err comparison of the parity tests, one JSON line each"""
+    path = os.environ.get("LBM_ERR_LOG")
+    if not path:
+        return
+    import json
+    # pytest imports the test file under its own name, other tests import
+    # it as tests.test_parity_gpu: two module objects, two logs
+    with open(path, "w") as fh:
+        for name in ("test_parity_gpu", "tests.test_parity_gpu"):
+            mod = sys.modules.get(name)
+            for test, rel, elem in getattr(mod, "ERR_LOG", []):
+                fh.write(json.dumps({"test": test, "rel": rel, "elem": elem}) + "\n")
