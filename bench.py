@@ -327,7 +327,7 @@ class _SharedTurbulence:
 
 
 def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_rms=0.05,
-                     nu=1e-4, k0=4, seed=0):
+                     nu=1e-4, k0=4, seed=0, params=None, weights=None):
     """D3Q27 decaying isotropic turbulence (E(k) ~ k^4 exp(-2 (k/k0)^2),
     u_rms 0.05, nu 1e-4 in lattice units, seeded field): MLUPS of plain KBC
     with FP32 storage and of the learned collision (seed-0 weights, KBC
@@ -336,7 +336,9 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
     against KBC with FP64 storage, every `every` steps over `steps` steps."""
     t_all = time.perf_counter()
     from lettuce_amd.ext._learned import TRAINED_D3Q27_KBC
-    if os.path.exists(TRAINED_D3Q27_KBC):
+    if params is not None:
+        weights = weights or "given parameters"
+    elif os.path.exists(TRAINED_D3Q27_KBC):
         params = lt.LearnedParameters.load(TRAINED_D3Q27_KBC)
         weights = ("trained on KBC decaying-turbulence states at 64^3 "
                    "(lettuce_amd/data/learned_d3q27_kbc.npz)")
@@ -374,8 +376,14 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
             # unbounded stable range
             a = lt.LearnedCollision(params)(flow)
             b = lt.LearnedCollision(free)(flow)
-            entry["kbc_fallback_cell_frac"] = float((a != b).any(0).float().mean())
-            del a, b
+            fb = (a != b).any(0)
+            entry["kbc_fallback_cell_frac"] = float(fb.float().mean())
+            # a wavefront of the learned kernel owns 64 consecutive cells
+            # along the contiguous axis and runs the KBC branch if any of
+            # them falls back
+            entry["kbc_fallback_wave_frac"] = float(
+                fb.reshape(-1, 64).any(1).float().mean())
+            del a, b, fb
         if name != "kbc_f64":
             gpu_s, _ = time_gpu(sim, timed, warmup)
             entry["mlups"] = round(timed * res ** 3 / 1e6 / gpu_s, 1)

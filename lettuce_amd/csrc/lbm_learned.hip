@@ -63,10 +63,6 @@ static uint16_t bf16_bits_rne(float x) {
   return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ float silu(float v) {
-  return v * __builtin_amdgcn_rcpf(1.0f + __expf(-v));
-}
-
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
   return *reinterpret_cast<const bf16x8*>(p);
 }
@@ -74,12 +70,21 @@ __device__ __forceinline__ f32x16 ld16(const float* p) {
   return *reinterpret_cast<const f32x16*>(p);
 }
 
-// SiLU of an accumulator, split into the two bf16 K chunks of the next layer
+// SiLU, silu(v) = v / (1 + exp(-v)) = v * rcp(1 + exp(-v)), of an
+// accumulator, split into the two bf16 K chunks of the next layer.
+// The 16 exponentials go out first, then the 16 reciprocals, then the
+// products: each transcendental's result is used 16 instructions later, so
+// no wait states sit between a transcendental and its use.
 __device__ __forceinline__ void activate(const f32x16& d, bf16x8& lo, bf16x8& hi) {
+  float r[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) r[t] = __expf(-d[t]);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) r[t] = __builtin_amdgcn_rcpf(1.0f + r[t]);
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    lo[t] = (__bf16)silu(d[t]);
-    hi[t] = (__bf16)silu(d[8 + t]);
+    lo[t] = (__bf16)(d[t] * r[t]);
+    hi[t] = (__bf16)(d[8 + t] * r[8 + t]);
   }
 }
 

@@ -411,8 +411,9 @@ class _StateFlow(lt.ExtFlow):
         self.f = self._f0
 
 
-def train3d(iters=150, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=0,
-            starts=(0, 100, 200, 300), lr=2e-3, eval_every=25, rate_hi=1.999, out=None):
+def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=0,
+            starts=(0, 200, 400, 600, 800, 1000), lr=2e-3, eval_every=25, rate_hi=1.999,
+            stable_hi=3.0, out=None):
     """Training of the D3Q27 learned collision on states that KBC generates.
 
     lettuce's DecayingTurbulence (D3Q27, E(k) ~ k^4 exp(-2 (k/k0)^2), u_rms
@@ -426,6 +427,11 @@ def train3d(iters=150, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
     `every` steps.  Held out: the same from the field of seed + 1.  The
     trained weights then run in the bf16 MFMA inference kernel on the
     held-out starts: loss and the fraction of cells that fall back to KBC.
+    The production limits clamp every rate into [0.05, rate_hi], inside the
+    stable interval (0, 2), and send a cell to KBC when its raw rate is NaN
+    or outside [0, stable_hi]: at tau = 0.5003 (1/tau = 1.9988) a fallback
+    bound of 2 would catch every raw rate a hair above 1/tau, and every
+    64-cell wavefront with one such cell runs both collisions.
     With `out`, the trained parameters are saved there (LearnedParameters.save)."""
     dev = torch.device("cuda:0")
     t_all = time.perf_counter()
@@ -493,9 +499,10 @@ def train3d(iters=150, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
         opt.step()
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    # the production limits: rates clamped below 2, the KBC fallback outside [0, 2]
+    # the production limits: rates clamped into [0.05, rate_hi], the KBC
+    # fallback for raw rates outside [0, stable_hi]
     trained = lt.unflatten_parameters(theta, like=base)
-    trained.stable_lo, trained.stable_hi = 0.0, 2.0
+    trained.stable_lo, trained.stable_hi = 0.0, stable_hi
     if out:
         trained.save(out)
 
@@ -520,7 +527,11 @@ def train3d(iters=150, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
 
     prod = {"kbc": production(lambda: lt.KBCCollision())[0]}
     for name, params in (("learned_initial", lt.LearnedParameters.seeded(seed, out_scale=0.1)),
-                         ("learned_trained", trained)):
+                         ("learned_trained", trained),
+                         ("learned_trained_fallback_above_2",
+                          lt.LearnedParameters(*trained.tensors(), in_scale=trained.in_scale,
+                                               rate_lo=trained.rate_lo, rate_hi=trained.rate_hi,
+                                               stable_lo=0.0, stable_hi=2.0))):
         loss_p, fb = production(lambda: lt.LearnedCollision(params))
         prod[name] = loss_p
         prod[name + "_fallback_cell_frac"] = fb

@@ -3,8 +3,10 @@ own trajectories (tests/golden) and the CPU oracle, in the pattern of
 lettuce's tests/native/*: build the same flow twice, step both, compare f.
 
 Tolerances: FP64 runs to 1e-12 absolute; FP32 runs to 1e-5 relative max
-error on f (max |f_gpu - f_ref| / max |f_ref|) after up to 1000 steps;
-masks, class maps and link lists bit-exact."""
+error on f (max |f_gpu - f_ref| / max |f_ref|) after up to 1000 steps, and
+to 5e-5 entrywise (max |f_gpu - f_ref| / |f_ref|, which weighs D3Q27's
+small corner populations fully); masks, class maps and link lists
+bit-exact."""
 import os
 
 import numpy as np
@@ -20,6 +22,10 @@ pytestmark = pytest.mark.gpu
 
 F32, F64 = torch.float32, torch.float64
 REL32 = 1e-5
+# the entrywise error (each population relative to itself) that the FP32
+# comparisons meet; measured on MI355X at most 2.5e-5 (1000 D3Q27 steps:
+# 2.1e-5, KBC golden: 2.5e-5, halfway channel: 8.6e-6)
+ELEM32 = 5e-5
 
 
 # (test, max-normalised error, entrywise error) of every rel_err comparison;
@@ -144,6 +150,7 @@ def test_tgv2d_1000_steps_matches_lettuce_fp32():
     out = native_run(flow, lt.BGKCollision(float(g["tau"])), 1000,
                      lt.StreamingStrategy.PRE_STREAMING)
     assert rel_err(out, g["f1000"]) <= REL32
+    assert elem_err(out, g["f1000"]) <= ELEM32
 
 
 def test_tgv2d_native_flow_setup_matches_lettuce():
@@ -166,6 +173,7 @@ def test_tgv3d_bgk_1000_steps_vs_oracle_fp32(name, n):
     flow = ArrayFlow(ctx(F32), f0, s)
     out = native_run(flow, lt.BGKCollision(tau), 1000, lt.StreamingStrategy.PRE_STREAMING)
     assert rel_err(out, oflow.f.numpy()) <= REL32
+    assert elem_err(out, oflow.f.numpy()) <= ELEM32
 
 
 @pytest.mark.parametrize("layout", ["rows", "soa"])
@@ -184,6 +192,7 @@ def test_tgv3d_two_step_kernel_1000_steps_vs_oracle_fp32(layout):
     assert sim._collide_and_stream.plan(1000) == (500, 0)
     sim(1000)
     assert rel_err(flow.f.cpu().numpy(), oflow.f.numpy()) <= REL32
+    assert elem_err(flow.f.cpu().numpy(), oflow.f.numpy()) <= ELEM32
 
 
 def test_tgv3d_golden_10_steps():
@@ -194,6 +203,7 @@ def test_tgv3d_golden_10_steps():
         out = native_run(flow, lt.BGKCollision(float(g[f"{name}_tau"])), 10,
                          lt.StreamingStrategy.PRE_STREAMING)
         assert rel_err(out, g[f"{name}_f10"]) <= REL32
+        assert elem_err(out, g[f"{name}_f10"]) <= ELEM32
 
 
 # --------------------------------------------------------------- TRT / KBC
@@ -217,9 +227,12 @@ def test_kbc_matches_lettuce(tag, dtype):
         post = lt.KBCCollision()(flow).cpu().numpy()
         ref = g[f"{name}_{tag}_coll"]
         tol = 1e-5 if dtype == F32 else 1e-12
+        etol = ELEM32 if dtype == F32 else 1e-12
         assert rel_err(post, ref) <= tol, name
+        assert elem_err(post, ref) <= etol, name
         out = native_run(flow, lt.KBCCollision(), 10, lt.StreamingStrategy.PRE_STREAMING)
         assert rel_err(out, g[f"{name}_{tag}_f10"]) <= tol, name
+        assert elem_err(out, g[f"{name}_{tag}_f10"]) <= etol, name
 
 
 @pytest.mark.parametrize("name,n", [("D2Q9", 64), ("D3Q27", 12)])
@@ -341,6 +354,7 @@ def test_halfway_channel_cylinder_vs_oracle(dtype, strategy):
     out = flow.f.cpu().numpy()
     tol = REL32 if dtype == F32 else 1e-12
     assert rel_err(out, oflow.f.numpy()) <= tol
+    assert elem_err(out, oflow.f.numpy()) <= (ELEM32 if dtype == F32 else 1e-12)
     if strategy == "PRE_STREAMING":
         # the FP64 link force of the stored post-collision state against the
         # momentum-exchange sum over the same links and populations (the
