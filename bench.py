@@ -291,18 +291,10 @@ class _SharedTurbulence:
     to every flow of the learned_vs_plain comparison"""
 
     def __init__(self, lt, res, u_rms, nu, k0, seed):
-        probe = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float64), [res] * 3, 1.0,
-                                      0.1, k0=k0, stencil=lt.D3Q27(),
-                                      initialize_pressure=False, initialize_fneq=False,
-                                      randseed=seed)
-        u = probe.u()
+        from lettuce_amd.studies import turbulence_units
         # lettuce scales the field to max |u| = Ma cs (decayingturbulence.py:
         # 167-169): Ma and Re follow from the requested u_rms and nu
-        ratio = float(torch.sqrt((u * u).sum(0).mean()) / torch.linalg.vector_norm(u, dim=0).max())
-        del probe, u
-        u_max = u_rms / ratio
-        self.ma = u_max / lt.D3Q27().cs
-        self.re = res * u_max / nu
+        self.re, self.ma = turbulence_units(res, u_rms, nu, k0, seed)
         flow = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float64), [res] * 3,
                                      self.re, self.ma, k0=k0, stencil=lt.D3Q27(),
                                      initialize_pressure=False, initialize_fneq=False,

@@ -171,24 +171,14 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
                  learned=True):
     """D3Q27 decaying isotropic turbulence, E(k) ~ k^4 exp(-2 (k/k0)^2),
     u_rms = 0.05 and nu = 1e-4 in lattice units: KBC with FP32 storage and
-    shifted BF16 / FP16 storage, and the learned collision (weights seeded
-    with `seed`, KBC fallback) with FP32 and shifted BF16 storage, each
-    against KBC in FP64: kinetic energy, the dissipation rate -dE/dt and the
-    enstrophy, every `every` steps."""
+    shifted BF16 / FP16 storage, and the learned collision (the weights
+    trained on KBC states, lettuce_amd/data/learned_d3q27_kbc.npz, KBC
+    fallback) with FP32 and shifted FP16 / BF16 storage, each against KBC in
+    FP64: kinetic energy, the dissipation rate -dE/dt and the enstrophy,
+    every `every` steps."""
     out = {"study": "turbulence3d_d3q27_kbc", "grid": [res] * 3, "steps": steps,
            "every": every, "nu_lu": nu, "u_rms_lu": u_rms}
-    # lettuce scales the field to max|u| = Ma cs (decayingturbulence.py:167-169):
-    # pick Ma and Re so that u_rms = 0.05 and nu = 1e-4 in lattice units
-    probe = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float64), [res] * 3, 1.0,
-                                  0.1, k0=k0, stencil=lt.D3Q27(), initialize_pressure=False,
-                                  initialize_fneq=False, randseed=seed)
-    u = probe.u()
-    ratio = float(torch.sqrt((u * u).sum(0).mean()) / torch.linalg.vector_norm(u, dim=0).max())
-    del probe, u
-    torch.cuda.empty_cache()
-    u_max = u_rms / ratio
-    ma = u_max / lt.D3Q27().cs
-    re = res * u_max / nu
+    re, ma = turbulence_units(res, u_rms, nu, k0, seed)
     out["ma"], out["re"] = ma, re
     series, timing = {}, {}
     runs = [("f64", torch.float64, "f64", "kbc"), ("f32", torch.float32, "f32", "kbc"),
@@ -196,14 +186,15 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
             ("f16_shifted", torch.float32, "f16_shifted", "kbc")]
     if learned:
         runs += [("learned_f32", torch.float32, "f32", "learned"),
+                 ("learned_f16_shifted", torch.float32, "f16_shifted", "learned"),
                  ("learned_bf16_shifted", torch.float32, "bf16_shifted", "learned")]
+        params = lt.LearnedParameters.load(lt.TRAINED_D3Q27_KBC)
     for name, dtype, storage, coll in runs:
         ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
         flow = lt.DecayingTurbulence(ctx, [res] * 3, re, ma, k0=k0, stencil=lt.D3Q27(),
                                      initialize_pressure=False, initialize_fneq=False,
                                      randseed=seed)
-        collision = (lt.KBCCollision() if coll == "kbc" else
-                     lt.LearnedCollision(lt.LearnedParameters.seeded(seed, out_scale=0.1)))
+        collision = lt.KBCCollision() if coll == "kbc" else lt.LearnedCollision(params)
         sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
         t0 = time.perf_counter()
         series[name] = _ke_series(sim, flow, steps, every)
@@ -411,9 +402,18 @@ class _StateFlow(lt.ExtFlow):
         self.f = self._f0
 
 
+def _enstrophy(u):
+    """mean |curl u|^2 of a periodic velocity [3, X, Y, Z], vorticity by
+    second-order central differences (differentiable)"""
+    def d(c, axis):
+        return 0.5 * (torch.roll(u[c], -1, axis) - torch.roll(u[c], 1, axis))
+    w = torch.stack([d(2, 1) - d(1, 2), d(0, 2) - d(2, 0), d(1, 0) - d(0, 1)])
+    return (w * w).sum(0).mean()
+
+
 def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=0,
             starts=(0, 200, 400, 600, 800, 1000), lr=2e-3, eval_every=25, rate_hi=1.999,
-            stable_hi=3.0, out=None):
+            stable_hi=3.0, enstrophy_weight=0.0, out=None):
     """Training of the D3Q27 learned collision on states that KBC generates.
 
     lettuce's DecayingTurbulence (D3Q27, E(k) ~ k^4 exp(-2 (k/k0)^2), u_rms
@@ -422,8 +422,10 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
     `starts` steps are the rollouts' starts and its states `steps` steps
     later the targets.  Model: `steps` steps of the learned collision's
     training formula from the same starts.  Loss: mean |u - u_KBC|^2 over the
-    cells, divided by the same for BGK (a zero MLP), averaged over the
-    starts.  Adam, the hand-written D3Q27 adjoint with checkpoints every
+    cells, divided by the same for BGK (a zero MLP), plus enstrophy_weight
+    times the squared error of the mean enstrophy (vorticity by central
+    differences), divided by BGK's, which weighs the dissipation at the
+    grid scale; averaged over the starts.  Adam, the hand-written D3Q27 adjoint with checkpoints every
     `every` steps.  Held out: the same from the field of seed + 1.  The
     trained weights then run in the bf16 MFMA inference kernel on the
     held-out starts: loss and the fraction of cells that fall back to KBC.
@@ -437,6 +439,8 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
     t_all = time.perf_counter()
     re, ma = turbulence_units(n, u_rms, nu, k0, seed)
 
+    ew = float(enstrophy_weight)
+
     def kbc_states(field_seed):
         flow = lt.DecayingTurbulence(lt.Context("cuda:0", dtype=torch.float32), [n] * 3, re,
                                      ma, k0=k0, stencil=lt.D3Q27(), initialize_pressure=False,
@@ -448,7 +452,8 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
             f0 = flow.f.contiguous().clone()
             sim(steps)
             t = s0 + steps
-            pairs.append((f0, flow.u().float().clone()))
+            ut = flow.u().float().clone()
+            pairs.append((f0, (ut, _enstrophy(ut))))
         return flow.units, pairs
 
     units, train = kbc_states(seed)
@@ -460,8 +465,17 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
     def vel(f):
         return torch.einsum("qd,q...->d...", e, f) / f.sum(0)
 
-    def err(u, ut):
-        return ((u - ut) ** 2).sum(0).mean()
+    def err(u, target):
+        """(velocity error, enstrophy error) against a target (u, enstrophy)"""
+        ut, wt = target
+        e_u = ((u - ut) ** 2).sum(0).mean()
+        if ew == 0.0:
+            return e_u, None
+        return e_u, (_enstrophy(u) - wt) ** 2
+
+    def ratio(e, b):
+        """the loss of an error pair against BGK's pair b"""
+        return e[0] / b[0] + (ew * e[1] / b[1] if ew else 0.0)
 
     base = lt.LearnedParameters.seeded(seed, out_scale=0.1, rate_hi=rate_hi,
                                        stable_lo=-1e30, stable_hi=1e30)
@@ -477,7 +491,7 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
 
     def test_loss(th):
         with torch.no_grad():
-            return float(sum(err(vel(lt.learned_rollout(f0, th, cfg, steps, every)), ut) / b
+            return float(sum(ratio(err(vel(lt.learned_rollout(f0, th, cfg, steps, every)), ut), b)
                              for (f0, ut), b in zip(test, bgk_te)) / len(test))
 
     torch.cuda.synchronize()
@@ -485,7 +499,7 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
     first = None
     for it in range(iters + 1):
         opt.zero_grad()
-        loss = sum(err(vel(lt.learned_rollout(f0, theta, cfg, steps, every)), ut) / b
+        loss = sum(ratio(err(vel(lt.learned_rollout(f0, theta, cfg, steps, every)), ut), b)
                    for (f0, ut), b in zip(train, bgk_tr)) / len(train)
         if first is None:
             first = float(loss)
@@ -522,7 +536,7 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
                 fallback += float((a != b_).any(0).float().mean())
             sim = lt.Simulation(flow, coll, [], lt.StreamingStrategy.PRE_STREAMING)
             sim(steps)
-            total += float(err(flow.u().float(), ut) / b)
+            total += float(ratio(err(flow.u().float(), ut), b))
         return total / len(test), fallback / len(test)
 
     prod = {"kbc": production(lambda: lt.KBCCollision())[0]}
@@ -537,10 +551,12 @@ def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=
         prod[name + "_fallback_cell_frac"] = fb
     return {"study": "train3d_d3q27_learned_kbc_targets", "grid": [n] * 3, "starts": list(starts),
             "steps": steps, "checkpoint_every": every, "tau": tau, "u_rms": u_rms, "nu": nu,
+            "enstrophy_weight": ew,
             "k0": k0, "iters": iters, "lr": lr, "rate_hi": rate_hi,
             "loss_initial": first, "loss_final_train": history[-1]["train"],
             "loss_final_test": history[-1]["test"],
-            "note": "losses are mean |u - u_KBC|^2 after `steps` steps divided by BGK's",
+            "note": "losses are mean |u - u_KBC|^2 (plus enstrophy_weight times the squared "
+                    "error of the mean enstrophy) after `steps` steps, each divided by BGK's",
             "production_kernels_test_loss": prod, "history": history,
             "train_s": round(t_train, 2), "ms_per_iter": round(1e3 * t_train / max(iters, 1), 2),
             "weights": out, "wall_s": round(time.perf_counter() - t_all, 2)}

@@ -186,3 +186,73 @@ def test_largest_d3q19_grid_keeps_its_translation_symmetry(layout):
     assert torch.isfinite(f[:, :h, :h]).all()
     for i, j in ((1, 0), (0, 1), (1, 1)):
         assert torch.equal(f[:, i * h:(i + 1) * h, j * h:(j + 1) * h], f[:, :h, :h]), (i, j)
+
+
+def _turbulence(ctx, res=384):
+    """the 384^3 D3Q27 decaying-turbulence workload: u_rms 0.05, nu 1e-4"""
+    from lettuce_amd.studies import turbulence_units
+    re, ma = turbulence_units(res)
+    return lt.DecayingTurbulence(ctx, [res] * 3, re, ma, k0=4, stencil=lt.D3Q27(),
+                                 initialize_pressure=False, initialize_fneq=False, randseed=0)
+
+
+def test_turbulence_384_kbc_and_trained_learned_at_full_size():
+    """The 384^3 D3Q27 decaying-turbulence workload at its real size: the
+    field has the requested u_rms and tau; KBC (FP32, two-step kernel)
+    conserves mass and momentum over 20 steps; the trained learned
+    collision with shifted FP16 storage conserves mass, leaves no cell on
+    the KBC fallback, and its kinetic energy tracks KBC's to 1e-3."""
+    flow = _turbulence(lt.Context("cuda:0", dtype=F32))
+    u = flow.u()
+    assert abs(float(torch.sqrt((u * u).sum(0).mean())) / 0.05 - 1) <= 1e-6
+    assert abs(flow.units.relaxation_parameter_lu - 0.5003) <= 1e-12
+    del u
+    sim = lt.Simulation(flow, lt.KBCCollision(), [], lt.StreamingStrategy.PRE_STREAMING)
+    assert sim._collide_and_stream.plan(20) == (10, 0)
+    s0 = _sums(flow)
+    sim(20)
+    s1 = _sums(flow)
+    cells = 384 ** 3
+    assert abs(s1[0] - s0[0]) <= 1e-6 * cells
+    assert np.all(np.abs(s1[1:4] - s0[1:4]) <= 1e-7 * cells)
+    ke_kbc = float(lt.IncompressibleKineticEnergy(flow)())
+    del sim, flow
+    torch.cuda.empty_cache()
+    params = lt.LearnedParameters.load(lt.TRAINED_D3Q27_KBC)
+    flow = _turbulence(lt.Context("cuda:0", dtype=F32, storage="f16_shifted"))
+    free = lt.LearnedParameters(*params.tensors(), in_scale=params.in_scale,
+                                rate_lo=params.rate_lo, rate_hi=params.rate_hi,
+                                stable_lo=-1e30, stable_hi=1e30)
+    assert torch.equal(lt.LearnedCollision(params)(flow), lt.LearnedCollision(free)(flow))
+    sim = lt.Simulation(flow, lt.LearnedCollision(params), [], lt.StreamingStrategy.PRE_STREAMING)
+    m0 = _sums(flow)[0]
+    sim(20)
+    assert abs(_sums(flow)[0] - m0) <= 1e-4 * cells
+    ke = float(lt.IncompressibleKineticEnergy(flow)())
+    assert abs(ke / ke_kbc - 1) <= 1e-3, (ke, ke_kbc)
+
+
+def test_channel_2048x512_post_streaming_split_and_forces_at_full_size(knobs):
+    """The 2048 x 512 channel with a D = 64 cylinder (halfway bounce-back on
+    the cylinder and walls, equilibrium inlet and outlet) at its real size,
+    KBC FP32, 200 steps in lettuce's default order: the split post-streaming
+    run equals push steps to the rounding of the collision, no NaN, and the
+    drag on the cylinder is positive and the same in both runs"""
+    nx, ny = 2048, 512
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+    mask = (xx - nx / 4) ** 2 + (yy - ny / 2 - 1) ** 2 < 32.0 ** 2
+    out, forces = [], []
+    for split in (0, 1):
+        knobs(post_split=split)
+        flow = lt.Obstacle(lt.Context("cuda:0", dtype=F32), [nx, ny], 200, 0.05 * 3 ** 0.5,
+                           domain_length_x=nx / 64, stencil=lt.D2Q9(), outlet="equilibrium",
+                           solid="halfway", walls=True)
+        flow.mask = mask
+        flow.initialize()
+        sim = lt.Simulation(flow, lt.KBCCollision(), [])
+        sim(200)
+        out.append(flow.f.cpu().numpy())
+        forces.append(flow.post_streaming_boundaries[0].force(sim).cpu().numpy())
+    assert np.isfinite(out[1]).all()
+    assert np.abs(out[0] - out[1]).max() <= 1e-4 * np.abs(out[0]).max()
+    assert forces[1][0] > 0 and abs(forces[1][0] / forces[0][0] - 1) <= 1e-3
