@@ -124,7 +124,10 @@ typedef struct lbm_cell_class {
  * x^a y^b z^c of order 3, of order 4, and of orders 5 and 6).
  * A cell whose raw r_g is NaN or outside [stable_lo, stable_hi] takes the
  * KBC update instead.  `packed` is the device copy of the
- * LBM_LEARNED_PACKED_BYTES buffer written by lbm_learned_pack. */
+ * LBM_LEARNED_PACKED_BYTES buffer written by lbm_learned_pack.
+ * The kernel evaluates the MLP with every pre-activation z scaled to
+ * -log2(e) z (silu becomes s / (1 + 2^s)), so the bf16 roundings of x and
+ * of the activations apply to the scaled values. */
 #define LBM_LEARNED_HIDDEN 32
 #define LBM_LEARNED_PACKED_BYTES 17408
 typedef struct lbm_learned {
@@ -305,8 +308,9 @@ int lbm_stream_copy(const void* src, void* dst, int64_t bytes,
 /* Host function: pack the MLP 6 -> 32 -> 32 -> 3 (row-major FP32 weights
  * w1[32][6], w2[32][32], w3[3][32] and biases b1[32], b2[32], b3[3]) into the
  * per-lane MFMA operand layout used by the learned-collision kernel; the
- * weights are rounded to bf16 (round to nearest even).  `out` is host memory
- * of LBM_LEARNED_PACKED_BYTES bytes; copy it to the device. */
+ * weights are rounded to bf16 (round to nearest even) and the biases are
+ * stored multiplied by -log2(e), for the kernel's scaled evaluation.  `out`
+ * is host memory of LBM_LEARNED_PACKED_BYTES bytes; copy it to the device. */
 int lbm_learned_pack(const float* w1, const float* b1, const float* w2,
                      const float* b2, const float* w3, const float* b3,
                      void* out);

@@ -129,6 +129,12 @@ struct StF32 {
   __device__ static C val(T x) { return x; }  // without the shift
   __device__ static C dec(T x, C) { return x; }
   __device__ static T enc(C v, C) { return v; }
+  // two populations at once (the 16-bit formats convert a pair in one
+  // instruction)
+  __device__ static void enc2(C a, C b, C, C, T (&e)[2]) {
+    e[0] = a;
+    e[1] = b;
+  }
   __device__ static C ld(const T* p, uint32_t i, C) { return elem(p, i); }
   __device__ static void st(T* p, uint32_t i, C v, C) {
     __builtin_nontemporal_store(v, elem_ptr(p, i));
@@ -153,6 +159,14 @@ struct StF16 {
   __device__ static C val(T x) { return (float)x; }  // without the shift
   __device__ static C dec(T x, C sh) { return (float)x + sh; }
   __device__ static T enc(C v, C sh) { return (_Float16)(v - sh); }
+  __device__ static void enc2(C a, C b, C sha, C shb, T (&e)[2]) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 v = {(_Float16)(a - sha), (_Float16)(b - shb)};
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    asm volatile("" : "+v"(u));
+    e[0] = __builtin_bit_cast(_Float16, (uint16_t)u);
+    e[1] = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+  }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
     return (float)elem(p, i) + sh;
   }
@@ -167,6 +181,15 @@ struct StBF16 {
   __device__ static C val(T x) { return bf16_to_f32(x); }  // without the shift
   __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
   __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
+  __device__ static void enc2(C a, C b, C sha, C shb, T (&e)[2]) {
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const b2 v = {(__bf16)(a - sha), (__bf16)(b - shb)};
+    // opaque: split into two halves, the pair would be converted one by one
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    asm volatile("" : "+v"(u));
+    e[0] = (uint16_t)u;
+    e[1] = (uint16_t)(u >> 16);
+  }
   __device__ static C ld(const T* p, uint32_t i, C sh) {
     return bf16_to_f32(elem(p, i)) + sh;
   }

@@ -70,15 +70,27 @@ __device__ __forceinline__ f32x16 ld16(const float* p) {
   return *reinterpret_cast<const f32x16*>(p);
 }
 
-// SiLU, silu(v) = v / (1 + exp(-v)) = v * rcp(1 + exp(-v)), of an
-// accumulator, split into the two bf16 K chunks of the next layer.
-// The 16 exponentials go out first, then the 16 reciprocals, then the
-// products: each transcendental's result is used 16 instructions later, so
-// no wait states sit between a transcendental and its use.
+// The MLP runs in the log2 domain: a layer's accumulator holds
+// s = -log2(e) z for the pre-activation z (the factor is folded into the
+// features' scale and into the biases), and
+//   s / (1 + 2^s) = s * rcp(1 + 2^s) = -log2(e) silu(z)
+// is the next layer's input in the same domain.  Since log2(e) ln(2) = 1 the
+// weights of layers 2 and 3 need no factor; the last layer's outputs are
+// multiplied by -ln(2).  This saves the multiplication of exp(-z) =
+// 2^(-log2(e) z) per activation; the bf16 roundings of the inputs and
+// activations happen on the scaled values.
+__host__ __device__ constexpr double kLog2e = 1.4426950408889634;
+__host__ __device__ constexpr double kLn2 = 0.6931471805599453;
+
+// SiLU of an accumulator in the log2 domain, split into the two bf16 K
+// chunks of the next layer.  The 16 exponentials go out first, then the 16
+// reciprocals, then the products: each transcendental's result is used 16
+// instructions later, so no wait states sit between a transcendental and
+// its use.
 __device__ __forceinline__ void activate(const f32x16& d, bf16x8& lo, bf16x8& hi) {
   float r[16];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) r[t] = __expf(-d[t]);
+  for (int t = 0; t < 16; ++t) r[t] = __builtin_amdgcn_exp2f(d[t]);
 #pragma unroll
   for (int t = 0; t < 16; ++t) r[t] = __builtin_amdgcn_rcpf(1.0f + r[t]);
 #pragma unroll
@@ -431,7 +443,7 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
       f32x16 dB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A30, hB0, c3, 0, 0, 0);
       dB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A31, hB1, dB, 0, 0, 0);
 #pragma unroll
-      for (int g = 0; g < 3; ++g) o[g] = lane < 32 ? dA[g] : dB[g];
+      for (int g = 0; g < 3; ++g) o[g] = (lane < 32 ? dA[g] : dB[g]) * (float)-kLn2;
     }
 
     const bool out_of_grid = i2 >= p.n2;  // never true: n2 % 64 == 0 is required
@@ -460,16 +472,42 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
     }
     if (out_of_grid) continue;
     if constexpr (!PUSH) {
-      // the cell's own slots through one per-lane pointer stepped by the
-      // population stride: the gather's per-population bases already fill
-      // the scalar registers, and 27 more would spill
+      // the cell's own slots: a scalar base, the tile's first cell of
+      // population q, stepped by the population stride, plus the lane's
+      // byte offset, which is the same for every population and tile, so
+      // no address takes a vector instruction.  The asm keeps the base in
+      // one scalar register pair, stepped between the stores: the gather's
+      // per-population bases already fill the scalar registers, and 27 more
+      // would spill.  16-bit formats convert two populations per
+      // instruction and store the second from the register's high half.
       using T = typename ST::T;
-      gptr<T> o = (gptr<T>)p.fout + (a0[1] + (long long)(a1[1] + a2[1]));
+      uint64_t sb = (uint64_t)p.fout +
+                    (uint64_t)(a0[1] + (long long)(a1[1] + b2 * 64)) * sizeof(T);
+      const uint64_t step = (uint64_t)p.sq * sizeof(T);
+      // opaque here, so that its zero extension is not hoisted out of the
+      // tile loop as a 64-bit value, which would take a 64-bit vector add
+      // per store instead of the 32-bit vector offset of the saddr form
+      uint32_t lane_off = (uint32_t)lane * (uint32_t)sizeof(T);
+      asm volatile("" : "+v"(lane_off));
+      auto slot = [&]() { return (gptr<T>)((gptr<char>)sb + lane_off); };
+      auto next = [&]() {
+        sb += step;
+        asm volatile("" : "+s"(sb));
+      };
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        __builtin_nontemporal_store(ST::enc(f[q], RAW ? C(0) : shift_of<ST>(S::w[q], p)), o);
-        o += p.sq;
-        asm volatile("" : "+v"(o));
+      for (int q = 0; q < Q; q += 2) {
+        const C sa = RAW ? C(0) : shift_of<ST>(S::w[q], p);
+        if (q + 1 < Q) {
+          const C sb1 = RAW ? C(0) : shift_of<ST>(S::w[q + 1], p);
+          T e[2];
+          ST::enc2(f[q], f[q + 1], sa, sb1, e);
+          __builtin_nontemporal_store(e[0], slot());
+          next();
+          __builtin_nontemporal_store(e[1], slot());
+          next();
+        } else {
+          __builtin_nontemporal_store(ST::enc(f[q], sa), slot());
+        }
       }
     } else {
       store_cell<S, ST, MASK, PUSH, RAW>(p, cell, a0, a1, a2, nsm, flags, f);
@@ -524,7 +562,7 @@ int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int stor
   if (!fn) return LBM_E_UNSUPPORTED;
   LearnedArgs la;
   la.packed = (const LearnedPacked*)l->packed;
-  la.in_scale = l->in_scale;
+  la.in_scale = (float)(-kLog2e * (double)l->in_scale);  // the log2 domain
   la.rate_lo = l->rate_lo;
   la.rate_hi = l->rate_hi;
   la.stable_lo = l->stable_lo;
@@ -583,10 +621,11 @@ extern "C" int lbm_learned_pack(const float* w1, const float* b1, const float* w
         P->a3[c][l][t] = bf16_bits_rne(w3p[i][krow(c, l, t)]);
       }
     }
+    // the biases in the log2 domain
     for (int r = 0; r < 16; ++r) {
-      P->c1[l][r] = b1[drow(l, r)];
-      P->c2[l][r] = b2[drow(l, r)];
-      P->c3[l][r] = b3p[drow(l, r)];
+      P->c1[l][r] = (float)(-kLog2e * (double)b1[drow(l, r)]);
+      P->c2[l][r] = (float)(-kLog2e * (double)b2[drow(l, r)]);
+      P->c3[l][r] = (float)(-kLog2e * (double)b3p[drow(l, r)]);
     }
   }
   return LBM_OK;

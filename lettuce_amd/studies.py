@@ -411,7 +411,7 @@ def _enstrophy(u):
     return (w * w).sum(0).mean()
 
 
-def train3d(iters=500, n=64, steps=16, every=4, u_rms=0.05, nu=1e-4, k0=4, seed=0,
+def train3d(iters=500, n=64, steps=64, every=8, u_rms=0.05, nu=1e-4, k0=4, seed=0,
             starts=(0, 200, 400, 600, 800, 1000), lr=2e-3, eval_every=25, rate_hi=1.999,
             stable_hi=3.0, enstrophy_weight=0.0, out=None):
     """Training of the D3Q27 learned collision on states that KBC generates.

@@ -861,6 +861,10 @@ def bf16_round(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
+LOG2E = 1.4426950408889634
+LN2 = 0.6931471805599453
+
+
 # D2Q9 dual vectors of the three higher-order raw moments of a population
 # vector with zero density, momentum and second moments:
 # m_xxy -> PHI_XXY, m_xyy -> PHI_XYY, m_xxyy -> PHI_XXYY
@@ -956,16 +960,26 @@ class LearnedCollision:
         self.kbc = KBCCollision()
         self.force_rates = None   # test hook: fixed rates [3] or [3, *res]
 
-    def mlp(self, x):
+    def mlp(self, x, rnd=bf16_round):
+        """o = W3 silu(W2 silu(W1 x + b1) + b2) + b3 with bf16 operands and
+        sums in x's dtype, evaluated as the inference kernel evaluates it: in
+        the log2 domain, where each pre-activation z is carried as
+        s = -log2(e) z, SiLU as s / (1 + 2^s) = -log2(e) silu(z) (so the
+        weights of layers 2 and 3 take no factor), and the output as -ln(2)
+        times its s.  The bf16 roundings (rnd) apply to the scaled inputs
+        and activations.  With rnd the identity it is the plain MLP up to
+        rounding."""
         p = self.params
-        h = torch.einsum("oi,i...->o...", p.w1, bf16_round(x)) + p.b1.reshape(
-            [-1] + [1] * (x.dim() - 1))
-        h = torch.nn.functional.silu(h)
-        h = torch.einsum("oi,i...->o...", p.w2, bf16_round(h)) + p.b2.reshape(
-            [-1] + [1] * (x.dim() - 1))
-        h = torch.nn.functional.silu(h)
-        return torch.einsum("oi,i...->o...", p.w3, bf16_round(h)) + p.b3.reshape(
-            [-1] + [1] * (x.dim() - 1))
+        shape = [-1] + [1] * (x.dim() - 1)
+
+        def layer(w, h, b):
+            bias = (-LOG2E * b.double()).to(x.dtype).reshape(shape)
+            return torch.einsum("oi,i...->o...", w.to(x.dtype), rnd(h)) + bias
+        s = layer(p.w1, x * -LOG2E, p.b1)
+        s = s / (1 + torch.exp2(s))
+        s = layer(p.w2, s, p.b2)
+        s = s / (1 + torch.exp2(s))
+        return -LN2 * layer(p.w3, s, p.b3)
 
     def inputs(self, flow, df, rho):
         if isinstance(flow.stencil, D3Q27):
