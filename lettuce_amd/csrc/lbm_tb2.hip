@@ -203,13 +203,26 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
       }
       collide<S, C, COLL>(g, cp);
       const long long plane = xb * p.s0;
+      // D3Q27: 27 loop-invariant store bases next to the 27 load bases
+      // would not fit the scalar registers, and the compiler would keep
+      // them in vector-register lanes, read back (v_readlane) in the load
+      // phase.  One base stepped by the population stride instead; the asm
+      // keeps each step in place.
+      constexpr bool STEP = S::Q == 27 && !WIDE;
+      uint64_t sb = (uint64_t)fout;
+      const uint64_t sqb = (uint64_t)p.sq * sizeof(T);
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) {
-        if constexpr (WIDE)
+        if constexpr (WIDE) {
           ST::st(scalar_base(fout + q * p.sq + plane), ay[1] + az[1], g[q],
                  shift_of<ST>(S::w[q], p));
-        else
+        } else if constexpr (STEP) {
+          ST::st((T*)sb, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
+          sb += sqb;
+          asm volatile("" : "+s"(sb));
+        } else {
           ST::st(fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
+        }
       }
       base_prio();
     }
