@@ -363,6 +363,20 @@ int lbm_learned_train_backward(int32_t stencil, int32_t storage, const int64_t n
 int lbm_unpull(int32_t stencil, int32_t storage, const int64_t n[3], const void* mu,
                void* lam, void* stream);
 
+/* Parameter gradients from a record of lbm_learned_train_backward (no
+ * lettuce counterpart), accumulated into grad [LBM_LEARNED_PARAMS] of the
+ * storage type (LBM_F32 or LBM_F64) in the parameter layout above:
+ *   dW1[i][j] += sum_c dz1_i(c) x_j(c)    db1[i] += sum_c dz1_i(c)
+ *   dW2[i][j] += sum_c dz2_i(c) a1_j(c)   db2[i] += sum_c dz2_i(c)
+ *   dW3[i][j] += sum_c do_i(c) a2_j(c)    db3[i] += sum_c do_i(c)
+ * over the record's `cells` columns.  workspace holds
+ * lbm_learned_grad_workspace(storage, cells) bytes of per-block partial
+ * sums, added in a fixed order (deterministic).  lbm_learned_grad_workspace
+ * returns -1 for an unsupported storage or cells < 1. */
+int64_t lbm_learned_grad_workspace(int32_t storage, int64_t cells);
+int lbm_learned_grad_reduce(int32_t storage, int64_t cells, const void* record,
+                            void* workspace, void* grad, void* stream);
+
 /* ---------------------------------------------------------------------------
  * x-slabs: one device's share of a lattice split along i0 (no lettuce
  * counterpart: lettuce steps the whole lattice on one device,

@@ -104,6 +104,8 @@ _SIGNATURES = {
     "lbm_learned_train_backward": ([_i32, _i32, _vp, _dbl, _vp, _dbl, _dbl, _dbl, _vp, _vp,
                                     _i32, _vp, _vp, _vp], ctypes.c_int),
     "lbm_unpull": ([_i32, _i32, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "lbm_learned_grad_workspace": ([_i32, _i64], _i64),
+    "lbm_learned_grad_reduce": ([_i32, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_halo_planes": ([_i32], _i64),
     "lbm_halo_pack": ([_i32, _i32, _I64_3, _I64_4, _vp, _vp, _vp, _vp], ctypes.c_int),
     "lbm_slab_step": ([ctypes.POINTER(LbmStepArgs), _i32, _vp, _vp, _vp, _vp, _vp],

@@ -598,6 +598,148 @@ __global__ void __launch_bounds__(256) unpull_kernel(const KParams p, const C* _
   for (int q = 0; q < S::Q; ++q) lam[q * p.sq + cell] = mu[q * p.sq + nb_offset<S>(q, +1, a0, a1, a2)];
 }
 
+// ---------------------------------------------------------------------------
+// Parameter gradients from the records of the backward kernel: sums over the
+// cells of products of record rows, in the parameter layout
+//   dW1[i][j] = sum_c dz1_i x_j    db1[i] = sum_c dz1_i
+//   dW2[i][j] = sum_c dz2_i a1_j   db2[i] = sum_c dz2_i
+//   dW3[i][j] = sum_c do_i a2_j    db3[i] = sum_c do_i
+// (1379 sums, against 67 x 70 for the full product of the record's two
+// halves).  Stage 1: block b sums cells [b * RG_CHUNK, (b + 1) * RG_CHUNK)
+// in tiles of RG_TILE cells staged in LDS, with a row of ones (for the
+// biases) and a row of zeros after the record's rows; LDS rows are padded
+// by one element, so that the rows the lanes of a wave read at one cell
+// fall in distinct banks.  Thread t holds dW2 rows 2 (t / 16) + u, columns
+// 2 (t % 16) + v (u, v < 2), and items t and t + 256 of the other 355 sums
+// (dW1, dW3, db1, db2, db3), each a product of two staged rows.  Stage 2
+// adds the blocks' partial sums to the gradient in block order, so the
+// result does not depend on scheduling.  (A GEMM over K = cells gives the
+// 67 x 70 product 25 output tiles of 16 x 16 for the whole device.)
+constexpr int RG_TILE = 32;
+constexpr int RG_CHUNK = 2048;
+constexpr int RG_ONE = NREC, RG_ZERO = NREC + 1, RG_ROWS = NREC + 2;
+constexpr int RG_REST = NPARAM - HID * HID;  // 355
+
+// the two staged rows whose product item r (< RG_REST) sums, and its index
+// in the parameter vector
+__device__ __forceinline__ void rest_item(int r, int& row, int& col, int& out) {
+  if (r < HID * 6) {  // dW1
+    row = R_DZ1 + r / 6;
+    col = R_X + r % 6;
+    out = P_W1 + r;
+  } else if (r < HID * 6 + 3 * HID) {  // dW3
+    const int q = r - HID * 6;
+    row = R_DO + q / HID;
+    col = R_A2 + q % HID;
+    out = P_W3 + q;
+  } else if (r < HID * 6 + 3 * HID + HID) {  // db1
+    const int q = r - HID * 6 - 3 * HID;
+    row = R_DZ1 + q;
+    col = RG_ONE;
+    out = P_B1 + q;
+  } else if (r < HID * 6 + 3 * HID + 2 * HID) {  // db2
+    const int q = r - HID * 6 - 4 * HID;
+    row = R_DZ2 + q;
+    col = RG_ONE;
+    out = P_B2 + q;
+  } else if (r < RG_REST) {  // db3
+    const int q = r - HID * 6 - 5 * HID;
+    row = R_DO + q;
+    col = RG_ONE;
+    out = P_B3 + q;
+  } else {  // no item
+    row = RG_ZERO;
+    col = RG_ZERO;
+    out = -1;
+  }
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) grad_partial_kernel(const C* __restrict__ rec, int64_t cells,
+                                                           C* __restrict__ part) {
+  __shared__ C s[RG_ROWS][RG_TILE + 1];
+  const int t = threadIdx.x;
+  const int r2 = R_DZ2 + 2 * (t >> 4), c2 = R_A1 + 2 * (t & 15);
+  int ra, ca, oa, rb, cb, ob;
+  rest_item(t, ra, ca, oa);
+  rest_item(t + 256, rb, cb, ob);
+  C g2[2][2] = {{C(0), C(0)}, {C(0), C(0)}}, ga = C(0), gb = C(0);
+  const int64_t c0 = (int64_t)blockIdx.x * RG_CHUNK;
+  const int64_t c1 = c0 + RG_CHUNK < cells ? c0 + RG_CHUNK : cells;
+  // the next tile's values are loaded into registers while the current
+  // tile is summed (element e = t + 256 i of a tile: row e / RG_TILE, cell
+  // e % RG_TILE)
+  constexpr int PER = (NREC * RG_TILE + 255) / 256;
+  C nxt[PER];
+  auto fetch = [&](int64_t base) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = t + 256 * i;
+      const int r = e / RG_TILE, k = e % RG_TILE;
+      const int64_t c = base + k;
+      nxt[i] = (e < NREC * RG_TILE && c < c1) ? rec[(int64_t)r * cells + c] : C(0);
+    }
+  };
+  fetch(c0);
+  for (int64_t base = c0; base < c1; base += RG_TILE) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = t + 256 * i;
+      if (e < NREC * RG_TILE) s[e / RG_TILE][e % RG_TILE] = nxt[i];
+    }
+    if (t < RG_TILE) {
+      s[RG_ONE][t] = base + t < c1 ? C(1) : C(0);
+      s[RG_ZERO][t] = C(0);
+    }
+    __syncthreads();
+    if (base + RG_TILE < c1) fetch(base + RG_TILE);
+#pragma unroll 8
+    for (int k = 0; k < RG_TILE; ++k) {
+      const C d0 = s[r2][k], d1 = s[r2 + 1][k], a0 = s[c2][k], a1 = s[c2 + 1][k];
+      g2[0][0] += d0 * a0;
+      g2[0][1] += d0 * a1;
+      g2[1][0] += d1 * a0;
+      g2[1][1] += d1 * a1;
+      ga += s[ra][k] * s[ca][k];
+      gb += s[rb][k] * s[cb][k];
+    }
+    __syncthreads();
+  }
+  C* dst = part + (int64_t)blockIdx.x * NPARAM;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+      dst[P_W2 + (r2 - R_DZ2 + u) * HID + (c2 - R_A1 + v)] = g2[u][v];
+  if (oa >= 0) dst[oa] = ga;
+  if (ob >= 0) dst[ob] = gb;
+}
+
+// grad[e] += sum over the nparts partial sums: thread group g (of 16) of a
+// block adds parts g, g + 16, g + 32, ..., and the groups' sums are
+// combined in group order
+constexpr int RG_GROUPS = 16;
+template <class C>
+__global__ void __launch_bounds__(64 * RG_GROUPS) grad_sum_kernel(const C* __restrict__ part,
+                                                                  int nparts, C* __restrict__ grad) {
+  __shared__ C red[RG_GROUPS][64];
+  const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + el;
+  C s = C(0);
+  if (e < NPARAM)
+    for (int b = g; b < nparts; b += RG_GROUPS) s += part[(int64_t)b * NPARAM + e];
+  red[g][el] = s;
+  __syncthreads();
+  if (g == 0 && e < NPARAM) {
+    C sum = red[0][el];
+#pragma unroll
+    for (int i = 1; i < RG_GROUPS; ++i) sum += red[i][el];
+    grad[e] += sum;
+  }
+}
+
+int64_t grad_parts(int64_t cells) { return (cells + RG_CHUNK - 1) / RG_CHUNK; }
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -715,6 +857,36 @@ int lbm_unpull(int32_t stencil, int32_t storage, const int64_t n[3], const void*
   if (mu == lam) return report_error(LBM_E_ARG, "mu and lam must differ");
   if (storage == LBM_F64) launch_unpull<double>(stencil, p, mu, lam, (hipStream_t)stream);
   else launch_unpull<float>(stencil, p, mu, lam, (hipStream_t)stream);
+  return launched();
+}
+
+int64_t lbm_learned_grad_workspace(int32_t storage, int64_t cells) {
+  if (storage != LBM_F32 && storage != LBM_F64) return -1;
+  if (cells < 1) return -1;
+  return grad_parts(cells) * NPARAM * (storage == LBM_F64 ? 8 : 4);
+}
+
+int lbm_learned_grad_reduce(int32_t storage, int64_t cells, const void* record, void* workspace,
+                            void* grad, void* stream) {
+  if (storage != LBM_F32 && storage != LBM_F64)
+    return report_error(LBM_E_UNSUPPORTED, "the gradient reduction takes F32 or F64 records");
+  if (cells < 1) return report_error(LBM_E_ARG, "cells must be positive");
+  if (!record || !workspace || !grad) return report_error(LBM_E_ARG, "null buffer");
+  const int64_t parts = grad_parts(cells);
+  if (parts > 0x7fffffff) return report_error(LBM_E_ARG, "too many cells");
+  const hipStream_t s = (hipStream_t)stream;
+  const unsigned sum_blocks = (NPARAM + 63) / 64;
+  if (storage == LBM_F64) {
+    hipLaunchKernelGGL(grad_partial_kernel<double>, dim3((unsigned)parts), dim3(256), 0, s,
+                       (const double*)record, cells, (double*)workspace);
+    hipLaunchKernelGGL(grad_sum_kernel<double>, dim3(sum_blocks), dim3(64 * RG_GROUPS), 0, s,
+                       (const double*)workspace, (int)parts, (double*)grad);
+  } else {
+    hipLaunchKernelGGL(grad_partial_kernel<float>, dim3((unsigned)parts), dim3(256), 0, s,
+                       (const float*)record, cells, (float*)workspace);
+    hipLaunchKernelGGL(grad_sum_kernel<float>, dim3(sum_blocks), dim3(64 * RG_GROUPS), 0, s,
+                       (const float*)workspace, (int)parts, (float*)grad);
+  }
   return launched();
 }
 

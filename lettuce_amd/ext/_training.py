@@ -125,6 +125,20 @@ class _Problem:
             record.data_ptr() if record is not None else None, self.stream),
             "lbm_learned_train_backward")
 
+    def grad_workspace(self):
+        """the per-block partial sums of lbm_learned_grad_reduce"""
+        nbytes = N.lib().lbm_learned_grad_workspace(self.storage, self.cells)
+        if nbytes < 0:
+            raise RuntimeError("lbm_learned_grad_workspace rejected the arguments")
+        return torch.empty(nbytes // self.theta.element_size(), dtype=self.theta.dtype,
+                           device=self.theta.device)
+
+    def grad_reduce(self, record, workspace, grad):
+        """grad [LEARNED_PARAMS] += the parameter gradient of one record"""
+        N.check(N.lib().lbm_learned_grad_reduce(self.storage, self.cells, record.data_ptr(),
+                                                workspace.data_ptr(), grad.data_ptr(),
+                                                self.stream), "lbm_learned_grad_reduce")
+
     def unpull(self, mu, lam):
         N.check(N.lib().lbm_unpull(self.stencil, self.storage, self.n, mu.data_ptr(), lam.data_ptr(),
                                    self.stream), "lbm_unpull")
@@ -156,9 +170,12 @@ class LearnedRollout(torch.autograd.Function):
         gathered = False
         mu = [torch.empty_like(lam), torch.empty_like(lam)]
         record = torch.empty((N.LEARNED_RECORD, prob.cells), dtype=lam.dtype, device=lam.device)
-        acc = torch.zeros((N.LEARNED_RECORD - _R_DZ1, _R_DZ1), dtype=lam.dtype, device=lam.device)
-        bias = torch.zeros(N.LEARNED_RECORD - _R_DZ1, dtype=lam.dtype, device=lam.device)
         need_theta = ctx.needs_input_grad[1]
+        # the parameter gradient, summed over the cells of every step's
+        # record by lbm_learned_grad_reduce
+        grad_theta = torch.zeros(N.LEARNED_PARAMS, dtype=lam.dtype, device=lam.device) \
+            if need_theta else None
+        workspace = prob.grad_workspace() if need_theta else None
         k = 0
         for seg in reversed(range(len(ctx.kept))):
             start = seg * every
@@ -172,24 +189,13 @@ class LearnedRollout(torch.autograd.Function):
                 out = mu[k & 1]
                 prob.adjoint(states[i], lam, gathered, out, record if need_theta else None)
                 if need_theta:
-                    # [dz1; dz2; do] x [x; a1; a2]^T holds the three weight
-                    # gradients as diagonal blocks
-                    acc.addmm_(record[_R_DZ1:], record[:_R_DZ1].t())
-                    bias.add_(record[_R_DZ1:].sum(1))
+                    prob.grad_reduce(record, workspace, grad_theta)
                 lam, gathered, k = out, True, k + 1
             states.clear()
         grad_f0 = None
         if ctx.needs_input_grad[0]:
             grad_f0 = torch.empty_like(lam)
             prob.unpull(lam, grad_f0)
-        grad_theta = None
-        if need_theta:
-            d1, d2, do = slice(0, H), slice(H, 2 * H), slice(2 * H, 2 * H + 3)
-            gw1 = acc[d1, _R_X:_R_A1]
-            gw2 = acc[d2, _R_A1:_R_A2]
-            gw3 = acc[do, _R_A2:_R_DZ1]
-            grad_theta = torch.cat([gw1.reshape(-1), bias[d1], gw2.reshape(-1), bias[d2],
-                                    gw3.reshape(-1), bias[do]])
         return grad_f0, grad_theta, None, None, None
 
 

@@ -96,6 +96,17 @@ def test_training_entry_points_reject_bad_arguments_without_a_gpu():
     mid = (ctypes.c_int64 * 3)(1 << 7, 1 << 10, 1 << 10)
     assert lib.lbm_unpull(D3, N.F32, mid, a, b, None) == -1
     assert b"2^31" in lib.lbm_last_error()
+    # the parameter-gradient reduction: one partial gradient per 2048
+    # cells, and argument checks before any HIP call
+    ws, P = lib.lbm_learned_grad_workspace, N.LEARNED_PARAMS
+    assert ws(N.F32, 2048) == P * 4
+    assert ws(N.F64, 2049) == 2 * P * 8
+    assert ws(N.F32, 32 * 256 * 256) == 1024 * P * 4
+    assert ws(N.BF16_SHIFTED, 2048) == -1 and ws(N.F32, 0) == -1
+    red = lib.lbm_learned_grad_reduce
+    assert red(N.F16_SHIFTED, 2048, a, b, p, None) == -3
+    assert red(N.F32, 0, a, b, p, None) == -1
+    assert red(N.F32, 2048, a, None, p, None) == -1
 
 
 def test_scan_rejects_bad_arguments_without_a_gpu():

@@ -31,6 +31,40 @@ def _oracle_rollout(f0, theta, tau, steps):
     return f
 
 
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-13), (F32, 1e-5)])
+@pytest.mark.parametrize("cells", [2048, 5000, 70000])
+def test_gradient_reduction_matches_the_products(dtype, tol, cells):
+    """lbm_learned_grad_reduce against the FP64 products on the CPU, in the
+    parameter layout [W1, b1, W2, b2, W3, b3]: dW1 = dz1 x^T, dW2 = dz2 a1^T,
+    dW3 = do a2^T and the biases the row sums of dz1, dz2, do, over the
+    cells of a random record, onto a non-zero gradient, for cell counts
+    that fill whole 2048-cell blocks, leave a partial block, and give many
+    blocks; two calls in a row with the same workspace add the sums twice"""
+    from lettuce_amd import _native as N
+    g = torch.Generator().manual_seed(cells)
+    rec = torch.randn(N.LEARNED_RECORD, cells, generator=g, dtype=F64).to(dtype)
+    grad0 = torch.randn(N.LEARNED_PARAMS, generator=g, dtype=F64).to(dtype)
+    r = rec.double()
+    x, a1, a2 = r[0:6], r[6:38], r[38:70]
+    dz1, dz2, do = r[70:102], r[102:134], r[134:137]
+    sums = torch.cat([(dz1 @ x.t()).reshape(-1), dz1.sum(1), (dz2 @ a1.t()).reshape(-1),
+                      dz2.sum(1), (do @ a2.t()).reshape(-1), do.sum(1)])
+    assert sums.numel() == N.LEARNED_PARAMS
+    storage = N.F64 if dtype == F64 else N.F32
+    lib = N.lib()
+    nbytes = lib.lbm_learned_grad_workspace(storage, cells)
+    ws = torch.empty(nbytes // rec.element_size(), dtype=dtype, device=DEV)
+    rec_d = rec.to(DEV).contiguous()
+    grad = grad0.to(DEV).contiguous()
+    stream = N.stream_handle(DEV)
+    for _ in range(2):
+        N.check(lib.lbm_learned_grad_reduce(storage, cells, rec_d.data_ptr(), ws.data_ptr(),
+                                            grad.data_ptr(), stream), "lbm_learned_grad_reduce")
+    ref = grad0.double() + 2 * sums
+    err = (grad.cpu().double() - ref).abs().max() / ref.abs().max()
+    assert err <= tol, err
+
+
 @pytest.mark.parametrize("steps", [1, 7])
 def test_forward_matches_oracle_f64(steps):
     tau = 0.58
