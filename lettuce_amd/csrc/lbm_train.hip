@@ -67,6 +67,12 @@ __device__ __forceinline__ C sigmoid(C z) {
   return C(1) / (C(1) + exp(-z));
 }
 
+// A scheduling barrier after each row of the MLP's matrix-vector products:
+// the weights come from LDS, and left alone the scheduler issues their
+// loads far ahead of their uses, which needed more than 256 VGPRs in the
+// backward kernel (spills to AGPRs and scratch, one wave per SIMD).
+__device__ __forceinline__ void row_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 // z1 = W1 x + b1, a1 = silu(z1), z2 = W2 a1 + b2, a2 = silu(z2),
 // raw = 1/tau + W3 a2 + b3, r = clamp(raw); with rec set, x, a1 and a2 go to
 // the record (stride rstride)
@@ -82,6 +88,7 @@ __device__ __forceinline__ void mlp_forward(const C* __restrict__ P, const C (&x
     for (int k = 0; k < 6; ++k) z += P[P_W1 + i * 6 + k] * x[k];
     z1[i] = z;
     a[i] = z * sigmoid(z);
+    row_fence();
   }
   if (rec) {
 #pragma unroll
@@ -95,6 +102,7 @@ __device__ __forceinline__ void mlp_forward(const C* __restrict__ P, const C (&x
 #pragma unroll
     for (int j = 0; j < HID; ++j) z += P[P_W2 + i * HID + j] * a[j];
     z2[i] = z;
+    row_fence();
   }
 #pragma unroll
   for (int i = 0; i < HID; ++i) a[i] = z2[i] * sigmoid(z2[i]);
@@ -109,6 +117,7 @@ __device__ __forceinline__ void mlp_forward(const C* __restrict__ P, const C (&x
     for (int j = 0; j < HID; ++j) o += P[P_W3 + k * HID + j] * a[j];
     raw[k] = tau_inv + o;
     r[k] = raw[k] < lo ? lo : (raw[k] > hi ? hi : raw[k]);
+    row_fence();
   }
 }
 
@@ -131,6 +140,7 @@ __device__ __forceinline__ void mlp_backward(const C* __restrict__ P, const C (&
 #pragma unroll
     for (int k = 0; k < 3; ++k) v += P[P_W3 + k * HID + i] * go[k];
     z2[i] = v * dsilu(z2[i]);
+    row_fence();
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) gx[k] = 0;
@@ -143,6 +153,7 @@ __device__ __forceinline__ void mlp_backward(const C* __restrict__ P, const C (&
     if (rec) rec[(R_DZ1 + j) * rstride] = gz1;
 #pragma unroll
     for (int k = 0; k < 6; ++k) gx[k] += P[P_W1 + j * 6 + k] * gz1;
+    row_fence();
   }
   if (rec) {
 #pragma unroll
@@ -539,9 +550,22 @@ __device__ __forceinline__ void adjoint_cell(const C (&g)[S::Q], const C (&lam)[
   else cell_backward27<C>(g, lam, P, tau_inv, s, lo, hi, mu, rec, rstride);
 }
 
+// The parameters, read by every cell, are staged in LDS once per block.
+// Read from global memory they are wave-uniform scalar loads; the 1379 of
+// them do not fit the scalar registers, and the compiler kept them in
+// vector-register lanes, read back with one v_readlane per use (15,616 of
+// the backward kernel's 23,831 instructions for D2Q9 FP32).
+template <class C>
+__device__ __forceinline__ void stage_params(const TrainArgs& t, C* sp) {
+  const C* P = (const C*)t.params;
+  for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sp[i] = P[i];
+  __syncthreads();
+}
+
 template <class S, class C>
 __global__ void __launch_bounds__(128) train_forward_kernel(const KParams p, const TrainArgs t) {
-  const C* P = (const C*)t.params;
+  __shared__ C P[NPARAM];
+  stage_params<C>(t, P);
   const C tau_inv = (C)p.tau_inv, s = (C)t.in_scale, lo = (C)t.rate_lo, hi = (C)t.rate_hi;
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= p.ncells) return;
@@ -561,12 +585,17 @@ __global__ void __launch_bounds__(128) train_forward_kernel(const KParams p, con
 
 // lam: the adjoint of the step's output, either directly (gathered == 0) or
 // as the previous backward step's mu, read at x + e_q (gathered == 1)
+// Two waves per SIMD: the kernel then fits 256 VGPRs with a few spills to
+// scratch (D2Q9 FP32: 212 bytes per lane); with one wave per SIMD and no
+// spills it ran slower (C4 training: 22.5 against 27.8 ms per iteration).
 template <class S, class C>
-__global__ void __launch_bounds__(128) train_backward_kernel(const KParams p, const TrainArgs t,
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+train_backward_kernel(const KParams p, const TrainArgs t,
                                                              const C* __restrict__ lam_in,
                                                              int gathered, C* __restrict__ mu_out,
                                                              C* __restrict__ rec) {
-  const C* P = (const C*)t.params;
+  __shared__ C P[NPARAM];
+  stage_params<C>(t, P);
   const C tau_inv = (C)p.tau_inv, s = (C)t.in_scale, lo = (C)t.rate_lo, hi = (C)t.rate_hi;
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= p.ncells) return;
