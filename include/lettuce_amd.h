@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 /* error codes */
 #define LBM_OK 0
@@ -164,6 +164,17 @@ typedef struct lbm_step_args {
   const lbm_learned* learned; /* host pointer to parameters, LEARNED only      */
   double force_tau;    /* BGK_GUO: tau of the source term (Guo's own tau)      */
   double accel[3];     /* BGK_GUO: acceleration in lattice units (x, y, z)     */
+  /* Stochastic rounding of LBM_BF16_SHIFTED storage (no lettuce counterpart):
+   * sr_seed == 0 rounds every stored value to nearest even; otherwise the 16
+   * bits the conversion drops are replaced by 16 random bits added before
+   * truncation, so a value rounds away from zero with a probability equal to
+   * the dropped fraction (unbiased on average).  The bits are a hash of
+   * sr_seed, the step index (sr_step for the first step of an lbm_run call,
+   * sr_step + k for step k), the stored element's offset and the population,
+   * so two-step launches equal two single steps bit for bit.  Other storage
+   * formats and x-slab steps take sr_seed == 0 only (LBM_E_UNSUPPORTED). */
+  uint32_t sr_seed;
+  uint32_t sr_step;
 } lbm_step_args;
 
 /* ABI version of the loaded library (LBM_ABI_VERSION it was built with). */

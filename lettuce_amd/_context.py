@@ -13,6 +13,12 @@ constructor arguments, defaults and tensor factories.  Differences:
   populations: ``"soa"`` (lettuce's contiguous [Q, *res]) or ``"rows"``
   (each row of the last axis stored for all Q populations together,
   [*res[:-1], Q, res[-1]]; periodic flows without boundaries only).
+* ``rounding`` selects how the step stores ``"bf16_shifted"`` populations:
+  ``"nearest"`` (round to nearest even, the default) or ``"stochastic"``
+  (the dropped bits replaced by random bits before truncation, unbiased on
+  average; the random bits follow from ``rounding_seed``, the step and the
+  element, so runs are reproducible).  Other storage formats round to
+  nearest.
 """
 from typing import List, Optional, Union
 
@@ -30,7 +36,8 @@ class Context:
     def __init__(self, device: Optional[Union[torch.device, str]] = None,
                  dtype: Optional[torch.dtype] = None,
                  use_native: Optional[bool] = None,
-                 storage: Optional[str] = None, layout: str = "soa"):
+                 storage: Optional[str] = None, layout: str = "soa",
+                 rounding: str = "nearest", rounding_seed: int = 0x2545F491):
         cuda = torch.cuda.is_available()
         if device is None:
             if use_native is None:
@@ -72,12 +79,23 @@ class Context:
 
         if layout not in ("soa", "rows"):
             raise ValueError(f"unknown layout {layout!r}; expected 'soa' or 'rows'")
+        if rounding not in ("nearest", "stochastic"):
+            raise ValueError(f"unknown rounding {rounding!r}; expected 'nearest' or "
+                             f"'stochastic'")
+        if rounding == "stochastic" and storage != "bf16_shifted":
+            raise ValueError("stochastic rounding applies to storage='bf16_shifted' only")
+        seed = int(rounding_seed) & 0xFFFFFFFF
+        if rounding == "stochastic" and seed == 0:
+            raise ValueError("rounding_seed must be non-zero")
 
         self.device = device
         self.dtype = dtype
         self.use_native = use_native
         self.storage = storage
         self.layout = layout
+        self.rounding = rounding
+        # 0 tells the kernels to round to nearest
+        self.rounding_seed = seed if rounding == "stochastic" else 0
 
     def population_tensor(self, q: int, resolution) -> torch.Tensor:
         """Uninitialised populations [q, *resolution] in the storage dtype,

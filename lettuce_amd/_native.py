@@ -21,7 +21,7 @@ __all__ = ["lib", "LbmStepArgs", "LbmCellClass", "LbmLearned", "check",
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "liblettuce_amd.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # enum values of include/lettuce_amd.h
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
@@ -68,7 +68,8 @@ class LbmStepArgs(ctypes.Structure):
                 ("tau_minus", ctypes.c_double), ("rho0_shift", ctypes.c_double),
                 ("learned", ctypes.POINTER(LbmLearned)),
                 ("force_tau", ctypes.c_double),
-                ("accel", ctypes.c_double * 3)]
+                ("accel", ctypes.c_double * 3),
+                ("sr_seed", ctypes.c_uint32), ("sr_step", ctypes.c_uint32)]
 
 
 _I64_3 = ctypes.c_int64 * 3

@@ -100,6 +100,7 @@ class NativeStep:
         a.force_tau = desc.force_tau
         a.accel[:] = list(desc.accel) + [0.0] * (3 - len(desc.accel))
         a.rho0_shift = flow.rho0_shift
+        a.sr_seed = getattr(ctx, "rounding_seed", 0)
         # the descriptor owns the learned struct and its packed weights; it
         # stays alive with this step (self.collision_desc)
         self._learned = desc.learned_struct()
@@ -134,6 +135,8 @@ class NativeStep:
             raise RuntimeError("f and f_next must share one layout")
         self.args.f_in = f_in.data_ptr()
         self.args.f_out = f_out.data_ptr()
+        # the index of the next step: stochastic rounding draws its bits from it
+        self.args.sr_step = int(flow.i) & 0xFFFFFFFF
 
     def __call__(self, simulation=None):
         self._bind_buffers()

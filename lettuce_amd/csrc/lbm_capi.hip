@@ -261,6 +261,8 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   if (a->collision == LBM_COLL_TRT && !(a->tau > 0 && a->tau_minus > 0))
     return fail(LBM_E_ARG, "TRT needs tau > 0 and tau_minus > 0");
   if (a->collision == LBM_COLL_KBC && !(a->tau > 0)) return fail(LBM_E_ARG, "KBC needs tau > 0");
+  if (a->sr_seed != 0 && a->storage != LBM_BF16_SHIFTED)
+    return fail(LBM_E_UNSUPPORTED, "stochastic rounding (sr_seed != 0) applies to BF16_SHIFTED storage only");
 
   fn = nullptr;
   if (a->collision == LBM_COLL_LEARNED) {
@@ -315,17 +317,22 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
   p.beta = (a->collision == LBM_COLL_KBC || learned) ? 1.0 / (2.0 * a->tau) : 0.0;
   p.beta_inv = (a->collision == LBM_COLL_KBC || learned) ? 2.0 * a->tau : 0.0;
   p.rho0 = (a->storage == LBM_F16_SHIFTED || a->storage == LBM_BF16_SHIFTED) ? a->rho0_shift : 0.0;
+  p.sr_seed = a->sr_seed;
+  p.sr_step = a->sr_step;
   grid = dim3((unsigned)nb);
   block = dim3(bs);
   return LBM_OK;
 }
 
-// `count` single-step launches of fn, alternating the buffers
+// `count` single-step launches of fn, alternating the buffers; `step` is
+// the index of the next step (stochastic rounding keys) and advances by one
+// per launch
 static void launch_singles(KernelFn fn, dim3 grid, dim3 block, KParams p, const void* const (&bufs)[2],
-                           int64_t& launch, int64_t count, hipStream_t s) {
+                           int64_t& launch, int64_t count, uint32_t& step, hipStream_t s) {
   for (int64_t k = 0; k < count; ++k, ++launch) {
     p.fin = bufs[launch & 1];
     p.fout = (void*)bufs[(launch + 1) & 1];
+    p.sr_step = step++;
     hipLaunchKernelGGL(fn, grid, block, 0, s, p);
   }
 }
@@ -336,7 +343,7 @@ static void launch_singles(KernelFn fn, dim3 grid, dim3 block, KParams p, const 
 // nsteps, so the result sits where single steps would leave it.
 static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn, dim3 grid,
                            dim3 block, const void* const (&bufs)[2], int64_t& launch,
-                           int64_t nsteps, hipStream_t s) {
+                           int64_t nsteps, uint32_t& step, hipStream_t s) {
   TbFn tb = two_step_kernel(a);
   const int64_t pairs = two_step_pairs(tb, nsteps);
   if (pairs > 0) {
@@ -373,10 +380,12 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
     for (int64_t k = 0; k < pairs; ++k, ++launch) {
       tp.k.fin = bufs[launch & 1];
       tp.k.fout = (void*)bufs[(launch + 1) & 1];
+      tp.k.sr_step = step;  // the launch's two steps: step and step + 1
+      step += 2;
       hipLaunchKernelGGL(tb, dim3((unsigned)nb), dim3(TB_THREADS), 0, s, tp);
     }
   }
-  launch_singles(fn, grid, block, p, bufs, launch, nsteps - 2 * pairs, s);
+  launch_singles(fn, grid, block, p, bufs, launch, nsteps - 2 * pairs, step, s);
 }
 
 // Whether lbm_run splits a post-streaming run (lettuce's default order,
@@ -461,6 +470,7 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   }
   const void* bufs[2] = {a->f_in, a->f_out};
   int64_t launch = 0;  // launches so far; launch k reads bufs[k & 1]
+  uint32_t step = a->sr_step;  // index of the next step (stochastic rounding)
   if (post_split(a, nsteps)) {
     // (S C)^n = (S C S) (C S)^(n-2) C: collide only, n - 2 pull steps (two
     // fused per launch where the two-step kernel applies), then stream,
@@ -475,13 +485,13 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
     if ((rc = prepare_step(&c, pc, fc, gc, bc)) || (rc = prepare_step(&k, pk, fk, gk, bk)) ||
         (rc = prepare_step(&d, pd, fd, gd, bd)))
       return rc;
-    launch_singles(fc, gc, bc, pc, bufs, launch, 1, s);
-    run_pull_steps(&k, pk, fk, gk, bk, bufs, launch, nsteps - 2, s);
-    launch_singles(fd, gd, bd, pd, bufs, launch, 1, s);
+    launch_singles(fc, gc, bc, pc, bufs, launch, 1, step, s);
+    run_pull_steps(&k, pk, fk, gk, bk, bufs, launch, nsteps - 2, step, s);
+    launch_singles(fd, gd, bd, pd, bufs, launch, 1, step, s);
     return check_launch("lbm_run (post-streaming)");
   }
-  if (a->streaming == LBM_PRE_STREAMING) run_pull_steps(a, p, fn, grid, block, bufs, launch, nsteps, s);
-  else launch_singles(fn, grid, block, p, bufs, launch, nsteps, s);
+  if (a->streaming == LBM_PRE_STREAMING) run_pull_steps(a, p, fn, grid, block, bufs, launch, nsteps, step, s);
+  else launch_singles(fn, grid, block, p, bufs, launch, nsteps, step, s);
   return check_launch("lbm_run");
 }
 
@@ -550,6 +560,7 @@ int lbm_slab_step(const lbm_step_args* a, int32_t part, const void* recv_lo, con
     return fail(LBM_E_UNSUPPORTED, "x-slabs use pull streaming (LBM_PRE_STREAMING)");
   if (a->collision == LBM_COLL_LEARNED)
     return fail(LBM_E_UNSUPPORTED, "x-slabs have no learned-collision kernel");
+  if (a->sr_seed != 0) return fail(LBM_E_UNSUPPORTED, "x-slabs round to nearest (sr_seed must be 0)");
   if (part < LBM_SLAB_ALL || part > LBM_SLAB_FACES) return fail(LBM_E_ARG, "unknown slab part %d", part);
   const bool faces = part != LBM_SLAB_INTERIOR;
   if (faces) {

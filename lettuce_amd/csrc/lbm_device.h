@@ -120,15 +120,50 @@ __device__ __forceinline__ T* scalar_base(T* p) {
   return (T*)v;
 }
 
+// Stochastic rounding of shifted BF16 (opt-in per run): the 16 bits that
+// the conversion to BF16 drops are replaced by 16 random bits added before
+// truncation, so a value rounds away from zero with a probability equal to
+// the dropped fraction and the rounding is unbiased on average.  Rounding to
+// nearest, applied step after step to smooth fields, biases the kinetic
+// energy.  The bits are a function of the run's seed, the step, the stored
+// element's offset and the population, so every kernel that stores the same
+// element at the same step draws the same bits.  Hash: lowbias32
+// (C. Wellons, "Prospecting for hash functions", 2018).
+__host__ __device__ constexpr uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+// uniform per launch: the key of step `step` of a run seeded with `seed`
+__host__ __device__ constexpr uint32_t sr_step_key(uint32_t seed, uint32_t step) {
+  return lowbias32(seed ^ lowbias32(step + 0x9e3779b9u));
+}
+// the element's 32 random bits at a step
+__device__ __forceinline__ uint32_t sr_cell(uint32_t key, uint32_t elem) {
+  return lowbias32(elem ^ key);
+}
+// population q's 16 bits from the element's hash: the high half of the
+// product with an odd constant per population
+__device__ __forceinline__ uint32_t sr_bits(uint32_t h, int q) {
+  return (h * (0x9e3779b1u + 0x3c6ef372u * (uint32_t)q)) >> 16;
+}
+
 // dec: stored value -> population, enc: population -> stored value;
-// ld / st: one element, stores bypassing the caches (streamed output)
+// enc_sr: the same with stochastic rounding from 16 random bits r (formats
+// other than BF16 ignore r); ld / st: one element, stores bypassing the
+// caches (streamed output)
 struct StF32 {
   using T = float;
   using C = float;
   static constexpr bool shifted = false;
+  static constexpr bool stochastic = false;
   __device__ static C val(T x) { return x; }  // without the shift
   __device__ static C dec(T x, C) { return x; }
   __device__ static T enc(C v, C) { return v; }
+  __device__ static T enc_sr(C v, C, uint32_t) { return v; }
   // two populations at once (the 16-bit formats convert a pair in one
   // instruction)
   __device__ static void enc2(C a, C b, C, C, T (&e)[2]) {
@@ -144,9 +179,11 @@ struct StF64 {
   using T = double;
   using C = double;
   static constexpr bool shifted = false;
+  static constexpr bool stochastic = false;
   __device__ static C val(T x) { return x; }  // without the shift
   __device__ static C dec(T x, C) { return x; }
   __device__ static T enc(C v, C) { return v; }
+  __device__ static T enc_sr(C v, C, uint32_t) { return v; }
   __device__ static C ld(const T* p, uint32_t i, C) { return p[i]; }
   __device__ static void st(T* p, uint32_t i, C v, C) {
     __builtin_nontemporal_store(v, p + i);
@@ -156,9 +193,11 @@ struct StF16 {
   using T = _Float16;
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr bool stochastic = false;
   __device__ static C val(T x) { return (float)x; }  // without the shift
   __device__ static C dec(T x, C sh) { return (float)x + sh; }
   __device__ static T enc(C v, C sh) { return (_Float16)(v - sh); }
+  __device__ static T enc_sr(C v, C sh, uint32_t) { return enc(v, sh); }
   __device__ static void enc2(C a, C b, C sha, C shb, T (&e)[2]) {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     const h2 v = {(_Float16)(a - sha), (_Float16)(b - shb)};
@@ -178,9 +217,19 @@ struct StBF16 {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr bool stochastic = true;
   __device__ static C val(T x) { return bf16_to_f32(x); }  // without the shift
   __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
   __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
+  // r < 2^16; infinities and NaN round to nearest (adding to their bits
+  // could turn one into the other), by a select, not a branch
+  __device__ static T enc_sr(C v, C sh, uint32_t r) {
+    const float x = v - sh;
+    const uint32_t u = __float_as_uint(x);
+    const uint16_t near = f32_to_bf16_rne(x);
+    const uint16_t sto = (uint16_t)((u + r) >> 16);
+    return (u & 0x7f800000u) == 0x7f800000u ? near : sto;
+  }
   __device__ static void enc2(C a, C b, C sha, C shb, T (&e)[2]) {
     typedef __bf16 b2 __attribute__((ext_vector_type(2)));
     const b2 v = {(__bf16)(a - sha), (__bf16)(b - shb)};

@@ -494,19 +494,31 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
         sb += step;
         asm volatile("" : "+s"(sb));
       };
+      if (ST::stochastic && p.sr_seed != 0) {
+        // stochastic rounding keyed to the step and this cell's offset
+        const uint32_t hs = sr_cell(sr_step_key(p.sr_seed, p.sr_step),
+                                    (uint32_t)(a0[1] + a1[1] + a2[1]));
 #pragma unroll
-      for (int q = 0; q < Q; q += 2) {
-        const C sa = RAW ? C(0) : shift_of<ST>(S::w[q], p);
-        if (q + 1 < Q) {
-          const C sb1 = RAW ? C(0) : shift_of<ST>(S::w[q + 1], p);
-          T e[2];
-          ST::enc2(f[q], f[q + 1], sa, sb1, e);
-          __builtin_nontemporal_store(e[0], slot());
+        for (int q = 0; q < Q; ++q) {
+          __builtin_nontemporal_store(
+              ST::enc_sr(f[q], RAW ? C(0) : shift_of<ST>(S::w[q], p), sr_bits(hs, q)), slot());
           next();
-          __builtin_nontemporal_store(e[1], slot());
-          next();
-        } else {
-          __builtin_nontemporal_store(ST::enc(f[q], sa), slot());
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q; q += 2) {
+          const C sa = RAW ? C(0) : shift_of<ST>(S::w[q], p);
+          if (q + 1 < Q) {
+            const C sb1 = RAW ? C(0) : shift_of<ST>(S::w[q + 1], p);
+            T e[2];
+            ST::enc2(f[q], f[q + 1], sa, sb1, e);
+            __builtin_nontemporal_store(e[0], slot());
+            next();
+            __builtin_nontemporal_store(e[1], slot());
+            next();
+          } else {
+            __builtin_nontemporal_store(ST::enc(f[q], sa), slot());
+          }
         }
       }
     } else {
@@ -591,6 +603,7 @@ int launch_learned(const KParams& p, const lbm_learned* l, int stencil, int stor
   for (int64_t k = 0; k < nsteps; ++k) {
     q.fin = bufs[k & 1];
     q.fout = (void*)bufs[(k + 1) & 1];
+    q.sr_step = p.sr_step + (uint32_t)k;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64 * waves_per_block), 0, stream, q, la);
   }
   return LBM_OK;

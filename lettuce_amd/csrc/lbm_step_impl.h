@@ -36,7 +36,23 @@ struct KParams {
   double accel[3], guo;  // BGK_GUO
   int x0 = 0;             // first i0 plane of the launch (slab interiors)
   int nclasses = 0;       // entries of ct (0 without boundaries)
+  uint32_t sr_seed = 0;   // stochastic rounding of BF16 storage (0: to nearest)
+  uint32_t sr_step = 0;   // step index of the launch (its first step)
 };
+
+// One stored element: to nearest, or with stochastic rounding when the
+// format supports it and the run asks for it (key: sr_step_key of the
+// step, elem: the element's offset, used for the random bits only).
+template <class ST>
+__device__ __forceinline__ void store_elem(const KParams& p, typename ST::T* base, uint32_t i,
+                                           typename ST::C v, typename ST::C sh, uint32_t key,
+                                           uint32_t elem, int q) {
+  if (ST::stochastic && p.sr_seed != 0)
+    __builtin_nontemporal_store(ST::enc_sr(v, sh, sr_bits(sr_cell(key, elem), q)),
+                                elem_ptr(base, i));
+  else
+    ST::st(base, i, v, sh);
+}
 
 // Which row (i0, i1) and which run b2 of the row a block takes: rows in
 // storage order, i1 fastest.  (XCD-aware orders, in which each of the 8
@@ -395,17 +411,20 @@ __device__ __forceinline__ void store_cell(const KParams& p, int cell, const X (
   using C = typename ST::C;
   using T = typename ST::T;
   T* fout = (T*)p.fout;
+  const uint32_t key = (ST::stochastic && p.sr_seed != 0) ? sr_step_key(p.sr_seed, p.sr_step) : 0u;
   if (!PUSH) {
+    const uint32_t own = (uint32_t)(a0[1] + a1[1] + a2[1]);
 #pragma unroll
     for (int q = 0; q < S::Q; ++q)
-      ST::st(at_plane(fout + q * p.sq, a0[1]), in_plane(a0[1], a1[1] + a2[1]), f[q],
-             RAW ? C(0) : shift_of<ST>(S::w[q], p));
+      store_elem<ST>(p, at_plane(fout + q * p.sq, a0[1]), in_plane(a0[1], a1[1] + a2[1]), f[q],
+                     RAW ? C(0) : shift_of<ST>(S::w[q], p), key, own, q);
   } else if (!MASK) {
 #pragma unroll
     for (int q = 0; q < S::Q; ++q)
-      ST::st(at_plane(fout + q * p.sq, nb_plane<S>(q, +1, a0)),
-             in_plane(nb_plane<S>(q, +1, a0), nb_in_plane<S>(q, +1, a1, a2)), f[q],
-             RAW ? C(0) : shift_of<ST>(S::w[q], p));
+      store_elem<ST>(p, at_plane(fout + q * p.sq, nb_plane<S>(q, +1, a0)),
+                     in_plane(nb_plane<S>(q, +1, a0), nb_in_plane<S>(q, +1, a1, a2)), f[q],
+                     RAW ? C(0) : shift_of<ST>(S::w[q], p), key,
+                     (uint32_t)(nb_plane<S>(q, +1, a0) + nb_in_plane<S>(q, +1, a1, a2)), q);
   } else {
     // Push with the pull rule evaluated at the destination d = x + e_q:
     //   d fluid and x solid (halfway bounce-back)  -> d fills itself
@@ -421,13 +440,13 @@ __device__ __forceinline__ void store_cell(const KParams& p, int cell, const X (
       const bool d_solid = (KD.flags & LBM_CLASS_SOLID) != 0;
       const bool d_keeps = ((KD.no_stream >> q) & 1u) != 0;
       const bool d_bounce = q != 0 && self_solid && !d_solid;
-      if (!d_keeps && !d_bounce) ST::st(fout + q * p.sq, dst, f[q], sh);
+      if (!d_keeps && !d_bounce) store_elem<ST>(p, fout + q * p.sq, dst, f[q], sh, key, dst, q);
       // (b) the cell's own slot q
       bool own_bounce = false;
       if (q != 0 && !self_solid && (flags & LBM_CLASS_NEAR_SOLID))
         own_bounce = cls_solid(p, nb_offset<S>(q, -1, a0, a1, a2));
-      if (own_bounce) ST::st(fout + q * p.sq, cell, f[S::opp[q]], sh);
-      else if ((nsm >> q) & 1u) ST::st(fout + q * p.sq, cell, f[q], sh);
+      if (own_bounce) store_elem<ST>(p, fout + q * p.sq, cell, f[S::opp[q]], sh, key, cell, q);
+      else if ((nsm >> q) & 1u) store_elem<ST>(p, fout + q * p.sq, cell, f[q], sh, key, cell, q);
     }
   }
 }

@@ -115,6 +115,10 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   const CollParams<C> cp = coll_params<C>(p);
   const T* fin = (const T*)p.fin;
   T* fout = (T*)p.fout;
+  // stochastic rounding (BF16 storage): the keys of the launch's two steps
+  const bool sr = ST::stochastic && p.sr_seed != 0;
+  const uint32_t key1 = sr ? sr_step_key(p.sr_seed, p.sr_step) : 0u;
+  const uint32_t key2 = sr ? sr_step_key(p.sr_seed, p.sr_step + 1) : 0u;
   C cur[S::Q], h1[S::Q], h2[S::Q];
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) h1[q] = h2[q] = C(0);
@@ -150,10 +154,13 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) cur[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
       collide<S, C, COLL>(cur, cp);
+      // S1 rounded to the storage format as a single step would store it,
+      // stochastic rounding keyed to the first step and this cell
+      const uint32_t hs = sr ? sr_cell(key1, (uint32_t)(ax[1] + ay[1] + az[1])) : 0u;
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) {
         const C sh = shift_of<ST>(S::w[q], p);
-        const T t = ST::enc(cur[q], sh);
+        const T t = sr ? ST::enc_sr(cur[q], sh, sr_bits(hs, q)) : ST::enc(cur[q], sh);
         cur[q] = ST::dec(t, sh);
         const int slot_of = it % tb_ring<S, NBAR>(q);
         if (!edge) {
@@ -211,17 +218,19 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
       constexpr bool STEP = S::Q == 27 && !WIDE;
       uint64_t sb = (uint64_t)fout;
       const uint64_t sqb = (uint64_t)p.sq * sizeof(T);
+      const uint32_t elem2 = (uint32_t)(plane + ay[1] + az[1]);
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) {
+        const C sh = shift_of<ST>(S::w[q], p);
         if constexpr (WIDE) {
-          ST::st(scalar_base(fout + q * p.sq + plane), ay[1] + az[1], g[q],
-                 shift_of<ST>(S::w[q], p));
+          store_elem<ST>(p, scalar_base(fout + q * p.sq + plane), ay[1] + az[1], g[q], sh, key2,
+                         elem2, q);
         } else if constexpr (STEP) {
-          ST::st((T*)sb, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
+          store_elem<ST>(p, (T*)sb, (int)plane + ay[1] + az[1], g[q], sh, key2, elem2, q);
           sb += sqb;
           asm volatile("" : "+s"(sb));
         } else {
-          ST::st(fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], shift_of<ST>(S::w[q], p));
+          store_elem<ST>(p, fout + q * p.sq, (int)plane + ay[1] + az[1], g[q], sh, key2, elem2, q);
         }
       }
       base_prio();

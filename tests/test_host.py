@@ -67,6 +67,11 @@ def test_step_rejects_bad_arguments_without_a_gpu():
     a.collision = N.COLL_BGK
     a.stride[:] = (10, 16, 4, 1)                        # populations overlap
     assert lib.lbm_step(ctypes.byref(a), None) == -1
+    a.stride[:] = (64, 16, 4, 1)
+    a.sr_seed = 12345                                   # stochastic rounding: BF16 only
+    assert lib.lbm_step(ctypes.byref(a), None) == -3
+    assert b"BF16" in lib.lbm_last_error()
+    a.sr_seed = 0
 
 
 def test_training_entry_points_reject_bad_arguments_without_a_gpu():
@@ -373,3 +378,19 @@ def test_poiseuille_units_and_acceleration_match_lettuce():
     assert units.relaxation_parameter_lu == float(g["tau"])
     a = units.convert_acceleration_to_lu(torch.tensor([0.001, 0.0], dtype=torch.float64))
     assert np.array_equal(np.asarray(a), g["acceleration_lu"])
+
+
+def test_stochastic_rounding_is_a_bf16_context_option():
+    """Context(rounding="stochastic") applies to shifted BF16 storage only
+    and carries a non-zero seed to the step; round to nearest carries 0."""
+    import pytest
+    import lettuce_amd as lt
+    c = lt.Context("cpu", storage="bf16_shifted", rounding="stochastic", rounding_seed=7)
+    assert c.rounding == "stochastic" and c.rounding_seed == 7
+    assert lt.Context("cpu", storage="bf16_shifted").rounding_seed == 0
+    with pytest.raises(ValueError):
+        lt.Context("cpu", storage="f16_shifted", rounding="stochastic")
+    with pytest.raises(ValueError):
+        lt.Context("cpu", storage="bf16_shifted", rounding="stochastic", rounding_seed=0)
+    with pytest.raises(ValueError):
+        lt.Context("cpu", rounding="up")
