@@ -47,10 +47,14 @@ WORKLOADS = {
     "tgv3d_d3q19_bgk_256_f32": ("D3Q19", "bgk", "f32", 256, 1600, 0.1),
     "tgv3d_d3q19_bgk_256_f16s": ("D3Q19", "bgk", "f16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_bgk_256_bf16s": ("D3Q19", "bgk", "bf16_shifted", 256, 1600, 0.1),
+    # names ending in _sr: BF16 storage with stochastic rounding
+    "tgv3d_d3q19_bgk_256_bf16s_sr": ("D3Q19", "bgk", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q27_kbc_256_f32": ("D3Q27", "kbc", "f32", 256, 1600, 0.1),
     "tgv3d_d3q27_kbc_256_bf16s": ("D3Q27", "kbc", "bf16_shifted", 256, 1600, 0.1),
+    "tgv3d_d3q27_kbc_256_bf16s_sr": ("D3Q27", "kbc", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q27_learned_256_f32": ("D3Q27", "learned", "f32", 256, 1600, 0.1),
     "tgv3d_d3q27_learned_256_bf16s": ("D3Q27", "learned", "bf16_shifted", 256, 1600, 0.1),
+    "tgv3d_d3q27_learned_256_bf16s_sr": ("D3Q27", "learned", "bf16_shifted", 256, 1600, 0.1),
     "tgv3d_d3q19_none_256_f32": ("D3Q19", "none", "f32", 256, 1600, 0.1),
     # lettuce's default streaming order (collide, then stream)
     "tgv3d_d3q19_bgk_256_f32_post": ("D3Q19", "bgk", "f32", 256, 1600, 0.1),
@@ -71,7 +75,8 @@ BYTES = {"f32": 4, "f64": 8, "f16_shifted": 2, "bf16_shifted": 2}
 def build_sim(lt, name, layout="soa"):
     stencil, coll, storage, res, re, ma = WORKLOADS[name]
     S = {"D2Q9": lt.D2Q9, "D3Q19": lt.D3Q19, "D3Q27": lt.D3Q27}[stencil]()
-    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage, layout=layout)
+    ctx = lt.Context("cuda:0", dtype=torch.float32, storage=storage, layout=layout,
+                     rounding="stochastic" if name.endswith("_sr") else "nearest")
     if name.startswith("cyl2d"):
         nx, ny = res
         flow = lt.Obstacle(ctx, [nx, ny], re, ma, domain_length_x=nx / 64, stencil=S,
@@ -323,7 +328,8 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
     """D3Q27 decaying isotropic turbulence (E(k) ~ k^4 exp(-2 (k/k0)^2),
     u_rms 0.05, nu 1e-4 in lattice units, seeded field): MLUPS of plain KBC
     with FP32 storage and of the learned collision (seed-0 weights, KBC
-    fallback) with shifted FP16 and BF16 storage, timed over `timed` steps
+    fallback) with shifted FP16 and BF16 storage (BF16 rounded to nearest
+    and stochastically), timed over `timed` steps
     after `warmup`; and the largest relative kinetic-energy error of each
     against KBC with FP64 storage, every `every` steps over `steps` steps."""
     t_all = time.perf_counter()
@@ -343,7 +349,8 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
     field = _SharedTurbulence(lt, res, u_rms, nu, k0, seed)
     runs = [("kbc_f64", torch.float64, "f64", "kbc"), ("kbc_f32", torch.float32, "f32", "kbc"),
             ("learned_f16s", torch.float32, "f16_shifted", "learned"),
-            ("learned_bf16s", torch.float32, "bf16_shifted", "learned")]
+            ("learned_bf16s", torch.float32, "bf16_shifted", "learned"),
+            ("learned_bf16s_sr", torch.float32, "bf16_shifted", "learned")]
     out = {"workload": f"turb3d_d3q27_{res}", "grid": [res] * 3, "cells": res ** 3,
            "u_rms_lu": u_rms, "nu_lu": nu, "ke_steps": steps, "ke_every": every,
            "timed_steps": timed, "reference": "kbc_f64", "learned_weights": weights,
@@ -351,7 +358,9 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
            "kbc_fallback_outside": [params.stable_lo, params.stable_hi]}
     ke = {}
     for name, dtype, storage, coll in runs:
-        flow = field.flow(lt, lt.Context("cuda:0", dtype=dtype, storage=storage))
+        rounding = "stochastic" if name.endswith("_sr") else "nearest"
+        flow = field.flow(lt, lt.Context("cuda:0", dtype=dtype, storage=storage,
+                                         rounding=rounding))
         out["tau"] = flow.units.relaxation_parameter_lu
         collision = lt.KBCCollision() if coll == "kbc" else lt.LearnedCollision(params)
         sim = lt.Simulation(flow, collision, [], lt.StreamingStrategy.PRE_STREAMING)
@@ -361,7 +370,7 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
             sim._collide_and_stream.run(every)
             series.append(float(energy()))
         ke[name] = np.array(series)
-        entry = {"storage": storage, "collision": coll.upper()}
+        entry = {"storage": storage, "collision": coll.upper(), "rounding": rounding}
         if coll == "learned":
             # cells that took the KBC fallback in the last state's collision:
             # where the learned collision differs from the same one with an
@@ -386,10 +395,10 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
         torch.cuda.empty_cache()
     ref = ke["kbc_f64"]
     out["kbc_f64"]["ke_final_over_initial"] = round(float(ref[-1] / ref[0]), 6)
-    for name in ("kbc_f32", "learned_f16s", "learned_bf16s"):
+    for name in ("kbc_f32", "learned_f16s", "learned_bf16s", "learned_bf16s_sr"):
         out[name]["ke_rel_err_max_vs_f64"] = float(np.abs(ke[name] / ref - 1).max())
     base = out["kbc_f32"]["mlups"]
-    for name in ("learned_f16s", "learned_bf16s"):
+    for name in ("learned_f16s", "learned_bf16s", "learned_bf16s_sr"):
         out[name]["speedup_vs_kbc_f32"] = round(out[name]["mlups"] / base, 3)
     out["wall_s"] = round(time.perf_counter() - t_all, 1)
     return out
@@ -478,6 +487,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "lattice": stencil,
                    "collision": coll.upper(), "storage": storage,
+                   "rounding": "stochastic" if args.workload.endswith("_sr") else "nearest",
                    "grid": sim.flow.resolution, "cells": cells,
                    "streaming": streaming, "Re": re, "Ma": ma,
                    "layout": args.layout,

@@ -64,12 +64,15 @@ def tgv2d(steps=1000, res=128):
 
 def tgv3d(steps=2000, res=256):
     """D3Q19 BGK TGV, Re 1600, Ma 0.1: FP32 storage and shifted FP16/BF16
-    storage (FP32 arithmetic) against FP64; KE and enstrophy every 50 steps."""
+    storage (FP32 arithmetic; BF16 rounded to nearest and stochastically)
+    against FP64; KE and enstrophy every 50 steps."""
     series, timing = {}, {}
-    for name, dtype, storage in (("f64", torch.float64, "f64"), ("f32", torch.float32, "f32"),
-                                 ("f16_shifted", torch.float32, "f16_shifted"),
-                                 ("bf16_shifted", torch.float32, "bf16_shifted")):
-        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+    for name, dtype, storage, rounding in (
+            ("f64", torch.float64, "f64", "nearest"), ("f32", torch.float32, "f32", "nearest"),
+            ("f16_shifted", torch.float32, "f16_shifted", "nearest"),
+            ("bf16_shifted", torch.float32, "bf16_shifted", "nearest"),
+            ("bf16_shifted_sr", torch.float32, "bf16_shifted", "stochastic")):
+        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage, rounding=rounding)
         flow = lt.TaylorGreenVortex(ctx, res, 1600, 0.1, lt.D3Q19(), initialize_fneq=False)
         sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
                             lt.StreamingStrategy.PRE_STREAMING)
@@ -81,7 +84,7 @@ def tgv3d(steps=2000, res=256):
         torch.cuda.empty_cache()
     ref = series["f64"]
     out = {"study": "tgv3d_d3q19_bgk", "grid": [res] * 3, "steps": steps, "every": 50}
-    for name in ("f32", "f16_shifted", "bf16_shifted"):
+    for name in ("f32", "f16_shifted", "bf16_shifted", "bf16_shifted_sr"):
         s = series[name]
         out[name] = {"ke_rel_err_max": float(np.max(np.abs(s[:, 1] / ref[:, 1] - 1))),
                      "enstrophy_rel_err_max": float(np.max(np.abs(s[:, 2] / ref[:, 2] - 1))),
@@ -173,9 +176,9 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
     u_rms = 0.05 and nu = 1e-4 in lattice units: KBC with FP32 storage and
     shifted BF16 / FP16 storage, and the learned collision (the weights
     trained on KBC states, lettuce_amd/data/learned_d3q27_kbc.npz, KBC
-    fallback) with FP32 and shifted FP16 / BF16 storage, each against KBC in
-    FP64: kinetic energy, the dissipation rate -dE/dt and the enstrophy,
-    every `every` steps."""
+    fallback) with FP32 and shifted FP16 / BF16 storage, BF16 also with
+    stochastic rounding (`_sr`), each against KBC in FP64: kinetic energy,
+    the dissipation rate -dE/dt and the enstrophy, every `every` steps."""
     out = {"study": "turbulence3d_d3q27_kbc", "grid": [res] * 3, "steps": steps,
            "every": every, "nu_lu": nu, "u_rms_lu": u_rms}
     re, ma = turbulence_units(res, u_rms, nu, k0, seed)
@@ -183,14 +186,17 @@ def turbulence3d(steps=5000, res=384, every=50, u_rms=0.05, nu=1e-4, k0=4, seed=
     series, timing = {}, {}
     runs = [("f64", torch.float64, "f64", "kbc"), ("f32", torch.float32, "f32", "kbc"),
             ("bf16_shifted", torch.float32, "bf16_shifted", "kbc"),
+            ("bf16_shifted_sr", torch.float32, "bf16_shifted", "kbc"),
             ("f16_shifted", torch.float32, "f16_shifted", "kbc")]
     if learned:
         runs += [("learned_f32", torch.float32, "f32", "learned"),
                  ("learned_f16_shifted", torch.float32, "f16_shifted", "learned"),
-                 ("learned_bf16_shifted", torch.float32, "bf16_shifted", "learned")]
+                 ("learned_bf16_shifted", torch.float32, "bf16_shifted", "learned"),
+                 ("learned_bf16_shifted_sr", torch.float32, "bf16_shifted", "learned")]
         params = lt.LearnedParameters.load(lt.TRAINED_D3Q27_KBC)
     for name, dtype, storage, coll in runs:
-        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage,
+                         rounding="stochastic" if name.endswith("_sr") else "nearest")
         flow = lt.DecayingTurbulence(ctx, [res] * 3, re, ma, k0=k0, stencil=lt.D3Q27(),
                                      initialize_pressure=False, initialize_fneq=False,
                                      randseed=seed)
