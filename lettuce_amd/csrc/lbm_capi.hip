@@ -214,7 +214,8 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
   // two, FP32 63.5k -> 63.1k
   const int barriers =
       knob(K_TB2_BARRIERS) > 0 ? knob(K_TB2_BARRIERS) : (storage_bytes(a->storage) == 2 ? 2 : 1);
-  return pick_tb2_kernel(a->stencil, a->storage, a->collision, barriers == 2 ? 2 : 1,
+  return pick_tb2_kernel(a->stencil, codec_id(a->storage, a->sr_seed), a->collision,
+                         barriers == 2 ? 2 : 1,
                          wide_offsets(a->storage, a->n, a->stride));
 }
 
@@ -278,10 +279,11 @@ static int prepare_step(const lbm_step_args* a, KParams& p, KernelFn& fn, dim3& 
     if (!(a->learned->rate_lo <= a->learned->rate_hi))
       return fail(LBM_E_ARG, "rate_lo must not exceed rate_hi");
   } else {
+    const int codec = codec_id(a->storage, a->sr_seed);
     switch (a->stencil) {
-      case LBM_D2Q9: fn = pick_kernel_d2q9(a->storage, a->collision, mask, a->streaming); break;
-      case LBM_D3Q19: fn = pick_kernel_d3q19(a->storage, a->collision, mask, a->streaming); break;
-      case LBM_D3Q27: fn = pick_kernel_d3q27(a->storage, a->collision, mask, a->streaming); break;
+      case LBM_D2Q9: fn = pick_kernel_d2q9(codec, a->collision, mask, a->streaming); break;
+      case LBM_D3Q19: fn = pick_kernel_d3q19(codec, a->collision, mask, a->streaming); break;
+      case LBM_D3Q27: fn = pick_kernel_d3q27(codec, a->collision, mask, a->streaming); break;
       default: fn = nullptr;
     }
     if (!fn)
@@ -463,7 +465,8 @@ int lbm_run(const lbm_step_args* a, int64_t nsteps, void* stream) {
   g_err[0] = 0;
   hipStream_t s = (hipStream_t)stream;
   if (a->collision == LBM_COLL_LEARNED) {
-    rc = launch_learned(p, a->learned, a->stencil, a->storage, a->cell_class != nullptr, a->streaming,
+    rc = launch_learned(p, a->learned, a->stencil, codec_id(a->storage, a->sr_seed),
+                        a->cell_class != nullptr, a->streaming,
                         wide_offsets(a->storage, a->n, a->stride), s, nsteps, a->f_in, a->f_out);
     if (rc) return fail(rc, "no learned kernel for storage %d", a->storage);
     return check_launch("lbm_run (learned)");

@@ -217,7 +217,7 @@ struct StBF16 {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
-  static constexpr bool stochastic = true;
+  static constexpr bool stochastic = false;
   __device__ static C val(T x) { return bf16_to_f32(x); }  // without the shift
   __device__ static C dec(T x, C sh) { return bf16_to_f32(x) + sh; }
   __device__ static T enc(C v, C sh) { return f32_to_bf16_rne(v - sh); }
@@ -246,6 +246,18 @@ struct StBF16 {
     __builtin_nontemporal_store(f32_to_bf16_rne(v - sh), elem_ptr(p, i));
   }
 };
+// BF16 storage whose kernels store with stochastic rounding (enc_sr with
+// bits keyed to the run's seed, the step and the element).  A codec of its
+// own, so the kernels that round to nearest carry none of its code.
+struct StBF16SR : StBF16 {
+  static constexpr bool stochastic = true;
+};
+// Kernel-table id of StBF16SR (the public storage ids are 0-3): what the
+// dispatchers take for BF16_SHIFTED storage with a non-zero sr_seed.
+constexpr int kCodecBF16SR = 16;
+__host__ __device__ inline int codec_id(int storage, uint32_t sr_seed) {
+  return storage == LBM_BF16_SHIFTED && sr_seed != 0 ? kCodecBF16SR : storage;
+}
 
 // ---------------------------------------------------------------------------
 // Moments and the quadratic equilibrium

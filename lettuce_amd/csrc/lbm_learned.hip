@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
         sb += step;
         asm volatile("" : "+s"(sb));
       };
-      if (ST::stochastic && p.sr_seed != 0) {
+      if constexpr (ST::stochastic) {
         // stochastic rounding keyed to the step and this cell's offset
         const uint32_t hs = sr_cell(sr_step_key(p.sr_seed, p.sr_step),
                                     (uint32_t)(a0[1] + a1[1] + a2[1]));
@@ -551,6 +551,7 @@ LearnedFn learned_storage(int storage, bool mask, int mode) {
     case LBM_F32: return learned_mask<S, StF32, WIDE>(mask, mode);
     case LBM_F16_SHIFTED: return learned_mask<S, StF16, WIDE>(mask, mode);
     case LBM_BF16_SHIFTED: return learned_mask<S, StBF16, WIDE>(mask, mode);
+    case kCodecBF16SR: return learned_mask<S, StBF16SR, WIDE>(mask, mode);
   }
   return nullptr;  // FP64 storage: the learned collision computes in FP32
 }

@@ -47,7 +47,7 @@ template <class ST>
 __device__ __forceinline__ void store_elem(const KParams& p, typename ST::T* base, uint32_t i,
                                            typename ST::C v, typename ST::C sh, uint32_t key,
                                            uint32_t elem, int q) {
-  if (ST::stochastic && p.sr_seed != 0)
+  if constexpr (ST::stochastic)
     __builtin_nontemporal_store(ST::enc_sr(v, sh, sr_bits(sr_cell(key, elem), q)),
                                 elem_ptr(base, i));
   else
@@ -411,7 +411,7 @@ __device__ __forceinline__ void store_cell(const KParams& p, int cell, const X (
   using C = typename ST::C;
   using T = typename ST::T;
   T* fout = (T*)p.fout;
-  const uint32_t key = (ST::stochastic && p.sr_seed != 0) ? sr_step_key(p.sr_seed, p.sr_step) : 0u;
+  const uint32_t key = ST::stochastic ? sr_step_key(p.sr_seed, p.sr_step) : 0u;
   if (!PUSH) {
     const uint32_t own = (uint32_t)(a0[1] + a1[1] + a2[1]);
 #pragma unroll
@@ -558,6 +558,7 @@ KernelFn pick_kernel(int storage, int coll, bool mask, int mode) {
     case LBM_F64: return pick_coll<S, StF64>(coll, mask, mode);
     case LBM_F16_SHIFTED: return pick_coll<S, StF16>(coll, mask, mode);
     case LBM_BF16_SHIFTED: return pick_coll<S, StBF16>(coll, mask, mode);
+    case kCodecBF16SR: return pick_coll<S, StBF16SR>(coll, mask, mode);
   }
   return nullptr;
 }

@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   const T* fin = (const T*)p.fin;
   T* fout = (T*)p.fout;
   // stochastic rounding (BF16 storage): the keys of the launch's two steps
-  const bool sr = ST::stochastic && p.sr_seed != 0;
+  constexpr bool sr = ST::stochastic;
   const uint32_t key1 = sr ? sr_step_key(p.sr_seed, p.sr_step) : 0u;
   const uint32_t key2 = sr ? sr_step_key(p.sr_seed, p.sr_step + 1) : 0u;
   C cur[S::Q], h1[S::Q], h2[S::Q];
@@ -270,6 +270,7 @@ TbFn pick_tb2(int storage, int coll, int barriers, bool wide) {
     case LBM_F32: return pick_tb2_coll<S, StF32>(coll, barriers, wide);
     case LBM_F16_SHIFTED: return pick_tb2_coll<S, StF16>(coll, barriers, wide);
     case LBM_BF16_SHIFTED: return pick_tb2_coll<S, StBF16>(coll, barriers, wide);
+    case kCodecBF16SR: return pick_tb2_coll<S, StBF16SR>(coll, barriers, wide);
   }
   return nullptr;
 }

@@ -9,8 +9,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _flow(storage="f32", layout="soa", d=2, res=32):
+    """storage "bf16_shifted_sr": BF16 with stochastic rounding"""
     dtype = torch.float64 if storage == "f64" else torch.float32
-    ctx = lt.Context("cuda:0", dtype=dtype, storage=storage, layout=layout)
+    sr = storage.endswith("_sr")
+    ctx = lt.Context("cuda:0", dtype=dtype, storage=storage[:-3] if sr else storage,
+                     layout=layout, rounding="stochastic" if sr else "nearest")
     stencil = lt.D2Q9() if d == 2 else lt.D3Q19()
     return lt.TaylorGreenVortex(ctx, res, 100, 0.1, stencil, initialize_fneq=False)
 
@@ -35,9 +38,12 @@ def test_dump_load_restores_f(tmp_path):
     assert torch.equal(flow.f, f0)
 
 
-@pytest.mark.parametrize("storage", ["f32", "f64", "f16_shifted", "bf16_shifted"])
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16_shifted", "bf16_shifted",
+                                     "bf16_shifted_sr"])
 @pytest.mark.parametrize("layout,d", [("soa", 2), ("rows", 3)])
 def test_restart_is_bit_exact(tmp_path, storage, layout, d):
+    """with stochastic rounding too: the random bits are keyed to the step
+    index the file restores"""
     res = 32 if d == 2 else 16
     a = _flow(storage, layout, d, res)
     sim = _sim(a)
