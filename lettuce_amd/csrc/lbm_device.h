@@ -145,10 +145,14 @@ __host__ __device__ constexpr uint32_t sr_step_key(uint32_t seed, uint32_t step)
 __device__ __forceinline__ uint32_t sr_cell(uint32_t key, uint32_t elem) {
   return lowbias32(elem ^ key);
 }
-// population q's 16 bits from the element's hash: the high half of the
-// product with an odd constant per population
+// population q's 16 bits from the element's hash: bits 16-31 of the product
+// of its low 24 bits with an odd 24-bit constant per population (distinct
+// for distinct q).  24-bit operands make the product one full-rate
+// v_mul_u32_u24; a 32-bit product is a quarter-rate instruction, and with
+// one per stored value it was the largest cost of the rounding.
 __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int q) {
-  return (h * (0x9e3779b1u + 0x3c6ef372u * (uint32_t)q)) >> 16;
+  const uint32_t c = ((0x9e3779b1u + 0x3c6ef372u * (uint32_t)q) & 0xffffffu) | 1u;
+  return ((h & 0xffffffu) * c) >> 16;
 }
 
 // dec: stored value -> population, enc: population -> stored value;
@@ -228,7 +232,8 @@ struct StBF16 {
     const uint32_t u = __float_as_uint(x);
     const uint16_t near = f32_to_bf16_rne(x);
     const uint16_t sto = (uint16_t)((u + r) >> 16);
-    return (u & 0x7f800000u) == 0x7f800000u ? near : sto;
+    // class mask: signalling and quiet NaN, -inf, +inf
+    return __builtin_amdgcn_classf(x, 0x207) ? near : sto;
   }
   __device__ static void enc2(C a, C b, C sha, C shb, T (&e)[2]) {
     typedef __bf16 b2 __attribute__((ext_vector_type(2)));
