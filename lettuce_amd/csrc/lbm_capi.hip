@@ -219,11 +219,36 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
                          wide_offsets(a->storage, a->n, a->stride));
 }
 
+// the two-step kernel for a 2D grid with boundaries, if one applies: D2Q9
+// with a cell-class map, pull streaming, a collision other than the learned
+// one, 32- or 16-bit storage, contiguous rows of a multiple of 64 and at
+// most 1024 cells, and the LDS ring within the device's per-block limit.
+// *lds: the block's dynamic LDS bytes, *ring_offset: the ring's offset.
+static Tb2dFn two_step_2d(const lbm_step_args* a, int64_t* lds, int64_t* ring_offset) {
+  if (knob(K_TWO_STEP) == 0) return nullptr;
+  if (a->stencil != LBM_D2Q9 || !a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
+  if (a->collision == LBM_COLL_LEARNED || a->storage == LBM_F64) return nullptr;
+  if (a->n[0] != 1 || a->n[2] % 64 != 0 || a->n[2] > 1024) return nullptr;
+  if (a->stride[3] != 1 || a->stride[2] != a->n[2]) return nullptr;
+  *lds = tb2d_lds_bytes((int)a->n_classes, (int)a->n[2], storage_bytes(a->storage), ring_offset);
+  int dev = 0, max_lds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+      *lds > max_lds)
+    return nullptr;
+  return pick_tb2d_kernel(a->stencil, codec_id(a->storage, a->sr_seed), a->collision);
+}
+
+static bool has_two_step(const lbm_step_args* a) {
+  int64_t lds = 0, off = 0;
+  return two_step_kernel(a) != nullptr || two_step_2d(a, &lds, &off) != nullptr;
+}
+
 // pairs of steps fused into one pass by the two-step kernel.  After nsteps
 // the result must sit in bufs[nsteps & 1] as with single steps, so an odd
 // number of pairs runs one pair as two single steps.
-static int64_t two_step_pairs(TbFn tb, int64_t nsteps) {
-  int64_t pairs = tb ? nsteps / 2 : 0;
+static int64_t two_step_pairs(bool two_step, int64_t nsteps) {
+  int64_t pairs = two_step ? nsteps / 2 : 0;
   if (pairs % 2 != 0) --pairs;
   return pairs;
 }
@@ -347,8 +372,41 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
                            dim3 block, const void* const (&bufs)[2], int64_t& launch,
                            int64_t nsteps, uint32_t& step, hipStream_t s) {
   TbFn tb = two_step_kernel(a);
-  const int64_t pairs = two_step_pairs(tb, nsteps);
-  if (pairs > 0) {
+  int64_t lds = 0, ring_offset = 0;
+  Tb2dFn t2 = tb ? nullptr : two_step_2d(a, &lds, &ring_offset);
+  const int64_t pairs = two_step_pairs(tb || t2, nsteps);
+  if (pairs > 0 && t2) {
+    Tb2dParams tp;
+    tp.k = p;
+    tp.ring_offset = (int)ring_offset;
+    // rows per block: the grid's rows over the blocks the device holds at
+    // once, at least 8 (each block computes S1 on 4 rows beyond its chunk)
+    const int n1 = (int)a->n[1], n2 = (int)a->n[2];
+    tp.cx = knob(K_TB2_PLANES);
+    if (tp.cx < 1) {
+      tp.cx = 8;
+      int per_cu = 0, dev = 0, cus = 0;
+      if (hipFuncSetAttribute((const void*)t2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds) == hipSuccess &&
+          hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)t2, n2, (size_t)lds) ==
+              hipSuccess &&
+          per_cu > 0) {
+        const int64_t resident = (int64_t)per_cu * cus;
+        tp.cx = (int)std::max<int64_t>(8, (n1 + resident - 1) / resident);
+      }
+    }
+    (void)hipFuncSetAttribute((const void*)t2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const unsigned nb = (unsigned)((n1 + tp.cx - 1) / tp.cx);
+    for (int64_t k = 0; k < pairs; ++k, ++launch) {
+      tp.k.fin = bufs[launch & 1];
+      tp.k.fout = (void*)bufs[(launch + 1) & 1];
+      tp.k.sr_step = step;  // the launch's two steps: step and step + 1
+      step += 2;
+      hipLaunchKernelGGL(t2, dim3(nb), dim3(n2), (size_t)lds, s, tp);
+    }
+  } else if (pairs > 0) {
     TbParams tp;
     tp.k = p;
     tp.nzt = (int)(a->n[2] / 64);
@@ -445,10 +503,10 @@ int lbm_run_plan(const lbm_step_args* a, int64_t nsteps, int64_t* plan) {
   } else if (post_split(a, nsteps)) {
     lbm_step_args k = *a;
     k.streaming = LBM_PRE_STREAMING;
-    plan[0] = two_step_pairs(two_step_kernel(&k), nsteps - 2);
+    plan[0] = two_step_pairs(has_two_step(&k), nsteps - 2);
     plan[1] = nsteps - 2 * plan[0];
   } else {
-    plan[0] = two_step_pairs(two_step_kernel(a), nsteps);
+    plan[0] = two_step_pairs(has_two_step(a), nsteps);
     plan[1] = nsteps - 2 * plan[0];
   }
   return LBM_OK;

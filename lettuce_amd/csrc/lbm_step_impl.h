@@ -337,11 +337,13 @@ __device__ __forceinline__ void outlet_neighbour(const KParams& p, const lbm_cel
 
 // The operator of a non-collision cell (one of LBM_OP_BOUNCE_BACK,
 // LBM_OP_EQUILIBRIUM, LBM_OP_OUTLET, LBM_OP_ANTI_BOUNCE_BACK, LBM_OP_KEEP)
-// on its populations f.
-template <class S, class ST, bool PULL>
-__device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op, uint32_t flags,
-                                                  int cell, int i0, int i1, int i2,
-                                                  typename ST::C (&f)[S::Q]) {
+// on its populations f.  neighbour(K, g) fills g with the populations of
+// the neighbour x - K.dir as the streaming step leaves them (OUTLET,
+// ANTI_BOUNCE_BACK).
+template <class S, class ST, class NB>
+__device__ __forceinline__ void apply_boundary_op_nb(const KParams& p, uint32_t op, uint32_t flags,
+                                                     int cell, typename ST::C (&f)[S::Q],
+                                                     NB&& neighbour) {
   using C = typename ST::C;
   if (op == LBM_OP_BOUNCE_BACK) {
     // fullway bounce-back: f_q <- f_opp(q) (bounce_back_boundary.py:17-18)
@@ -370,7 +372,7 @@ __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op,
     // f <- feq(rho_out, u(x - dir)) (equilibrium_outlet_p.py:63-73)
     const lbm_cell_class& K = p.ct[p.cls[cell]];
     C g[S::Q];
-    outlet_neighbour<S, ST, PULL>(p, K, i0, i1, i2, g);
+    neighbour(K, g);
     C rho_n, u[3];
     moments<S, C>(g, rho_n, u);
     equilibrium<S, C>((C)K.rho, u, f);
@@ -381,7 +383,7 @@ __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op,
     // with u_w = u + (u - u(x - dir)) / 2 and rho, u of the cell itself
     const lbm_cell_class& K = p.ct[p.cls[cell]];
     C g[S::Q];
-    outlet_neighbour<S, ST, PULL>(p, K, i0, i1, i2, g);
+    neighbour(K, g);
     C rho, u[3], rho_n, un[3];
     moments<S, C>(f, rho, u);
     moments<S, C>(g, rho_n, un);
@@ -399,6 +401,16 @@ __device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op,
     }
   }
   // LBM_OP_KEEP: unchanged
+}
+
+template <class S, class ST, bool PULL>
+__device__ __forceinline__ void apply_boundary_op(const KParams& p, uint32_t op, uint32_t flags,
+                                                  int cell, int i0, int i1, int i2,
+                                                  typename ST::C (&f)[S::Q]) {
+  apply_boundary_op_nb<S, ST>(p, op, flags, cell, f,
+                              [&](const lbm_cell_class& K, typename ST::C(&g)[S::Q]) {
+                                outlet_neighbour<S, ST, PULL>(p, K, i0, i1, i2, g);
+                              });
 }
 
 // Write the cell's post-collision populations: in place (no post-streaming)

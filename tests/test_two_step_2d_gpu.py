@@ -1,0 +1,184 @@
+"""The two-step kernel for 2D grids with boundaries (csrc/lbm_tb2d.hip):
+D2Q9 channels past a cylinder with every boundary rule the masked step has
+(equilibrium inlet, equilibrium and anti-bounce-back outlets, halfway and
+fullway bounce-back, walls) against single masked steps, to the rounding of
+the collision and boundary arithmetic (the compiler may contract the same
+formulas into FMAs differently in the two kernels); and two-step runs
+against the oracle (lettuce's torch step).  Cases cover a partial last row
+chunk, chunks of one to sixteen rows, rows of 64 to 512 cells, the storage
+formats, and lettuce's default post-streaming order (whose pull steps run
+on the same kernel)."""
+import numpy as np
+import pytest
+import torch
+
+import lettuce_amd as lt
+from oracle import lettuce_torch as O
+
+pytestmark = pytest.mark.gpu
+
+F32, F64 = torch.float32, torch.float64
+PRE, POST = lt.StreamingStrategy.PRE_STREAMING, lt.StreamingStrategy.POST_STREAMING
+
+# largest difference of two decoded populations allowed per step for a
+# rounding difference: a few units in the last place of the stored format
+# at |f| <= 0.5 (FP32) or at the stored deviation |f - w| <= 0.1
+ULPS = {"f32": 4 * 0.5 * 2.0 ** -23, "f16_shifted": 4 * 0.1 * 2.0 ** -10,
+        "bf16_shifted": 4 * 0.1 * 2.0 ** -7}
+COLLISIONS = {"bgk": lambda tau: lt.BGKCollision(tau), "kbc": lambda tau: lt.KBCCollision(),
+              "trt": lambda tau: lt.TRTCollision(tau, 0.8), "none": lambda tau: lt.NoCollision()}
+
+
+def _mask(nx, ny):
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+    return (xx - nx / 4) ** 2 + (yy - ny / 2 + 0.5) ** 2 < (ny / 6) ** 2
+
+
+def _channel(storage, nx, ny, outlet, solid, walls=True, dtype=F32):
+    ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+    flow = lt.Obstacle(ctx, [nx, ny], 100, 0.1, domain_length_x=nx / 10, stencil=lt.D2Q9(),
+                       outlet=outlet, solid=solid, walls=walls)
+    flow.mask = _mask(nx, ny)
+    flow.initialize()
+    # a perturbed state, so that every population and rule contributes
+    g = torch.Generator().manual_seed(nx * 1000 + ny)
+    f = flow.f.double().cpu()
+    f = f * (1 + 0.02 * (torch.rand(f.shape, generator=g, dtype=F64) - 0.5))
+    flow.f = f.to(dtype)
+    return flow
+
+
+def _run(storage, nx, ny, outlet, solid, coll, steps, strategy, two_step, knobs, planes=0):
+    knobs(two_step=1 if two_step else 0, two_step_planes=planes)
+    # halfway walls with the equilibrium outlet (an outlet's neighbour
+    # column holds fluid or solid cells only, not fullway bounce-back walls
+    # or an anti-bounce-back outlet's walls)
+    flow = _channel(storage, nx, ny, outlet, solid,
+                    walls=outlet == "equilibrium" and solid == "halfway")
+    sim = lt.Simulation(flow, COLLISIONS[coll](flow.units.relaxation_parameter_lu), [], strategy)
+    plan = sim._collide_and_stream.plan(steps)
+    sim(steps)
+    torch.cuda.synchronize()
+    return flow.f.double().cpu().numpy(), plan
+
+
+CASES = [
+    # storage, nx, ny, outlet, solid, collision, rows per block (0: default)
+    ("f32", 100, 64, "equilibrium", "halfway", "bgk", 0),
+    ("f32", 96, 128, "antibounceback", "fullway", "bgk", 0),
+    ("f32", 70, 64, "equilibrium", "halfway", "kbc", 3),
+    ("f32", 64, 192, "antibounceback", "halfway", "trt", 16),
+    ("f32", 48, 64, "equilibrium", "fullway", "none", 1),
+    ("f32", 130, 512, "equilibrium", "halfway", "bgk", 0),
+    ("f16_shifted", 100, 64, "equilibrium", "halfway", "kbc", 0),
+    ("bf16_shifted", 100, 128, "antibounceback", "halfway", "bgk", 5),
+]
+
+
+@pytest.mark.parametrize("storage,nx,ny,outlet,solid,coll,planes", CASES)
+@pytest.mark.parametrize("strategy", [PRE, POST])
+def test_two_step_2d_matches_single_steps(storage, nx, ny, outlet, solid, coll, planes, strategy,
+                                          knobs):
+    for steps in (2, 5, 8):
+        ref, plan0 = _run(storage, nx, ny, outlet, solid, coll, steps, strategy, False, knobs)
+        assert plan0 == (0, steps)
+        got, plan = _run(storage, nx, ny, outlet, solid, coll, steps, strategy, True, knobs,
+                         planes)
+        pull = steps - 2 if strategy == POST else steps
+        pairs = pull // 2 - (pull // 2) % 2 if pull >= 2 else 0
+        assert plan == (pairs, steps - 2 * pairs), (steps, plan)
+        err = np.abs(got - ref).max()
+        tol = steps * ULPS[storage] * (50 if coll == "kbc" else 1)
+        assert err <= tol, (steps, err, tol)
+
+
+def test_two_step_2d_is_used_for_the_channel_workload():
+    """2048 x 512 FP32 channel with BGK: every pair of steps on the kernel"""
+    flow = _channel("f32", 2048, 512, "equilibrium", "halfway")
+    sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [], PRE)
+    assert sim._collide_and_stream.plan(100) == (50, 0)
+    assert sim._collide_and_stream.plan(6) == (2, 2)
+
+
+def test_grids_the_kernel_does_not_take_run_single_steps():
+    """rows that are not a multiple of 64 cells, FP64 storage: single steps"""
+    for storage, dtype, ny in (("f32", F32, 40), (None, F64, 64)):
+        ctx = lt.Context("cuda:0", dtype=dtype, storage=storage)
+        flow = lt.Obstacle(ctx, [64, ny], 100, 0.1, domain_length_x=6.4, stencil=lt.D2Q9(),
+                           outlet="equilibrium", solid="halfway", walls=True)
+        flow.mask = _mask(64, ny)
+        flow.initialize()
+        sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [], PRE)
+        assert sim._collide_and_stream.plan(8) == (0, 8)
+
+
+@pytest.mark.parametrize("strategy", ["PRE_STREAMING", "POST_STREAMING"])
+def test_two_step_2d_channel_vs_oracle(strategy):
+    """FP32 channel with a cylinder (equilibrium inlet and outlet, halfway
+    bounce-back on the cylinder and the walls), 128 x 64, 200 steps on the
+    two-step kernel, against the oracle in FP64: 1e-5 relative"""
+    nx, ny = 128, 64
+    mask = _mask(nx, ny)
+    oflow = O.ObstacleChannel(O.Context("cpu", F64), [nx, ny], 100, 0.1,
+                              domain_length_x=nx / 10.0, mask=mask, stencil=O.D2Q9())
+    f0 = oflow.f.numpy().copy()
+    osim = O.Simulation(oflow, O.BGKCollision(oflow.units.relaxation_parameter_lu), [],
+                        getattr(O.StreamingStrategy, strategy))
+    flow = lt.Obstacle(lt.Context("cuda:0", dtype=F32), [nx, ny], 100, 0.1,
+                       domain_length_x=nx / 10.0, stencil=lt.D2Q9(), outlet="equilibrium",
+                       solid="halfway", walls=True)
+    flow.mask = mask
+    flow.f = torch.from_numpy(f0).to(F32)
+    sim = lt.Simulation(flow, lt.BGKCollision(flow.units.relaxation_parameter_lu), [],
+                        getattr(lt.StreamingStrategy, strategy))
+    assert sim._collide_and_stream.plan(200)[0] > 0
+    osim(200)
+    sim(200)
+    out = flow.f.double().cpu().numpy()
+    ref = oflow.f.numpy()
+    assert np.abs(out - ref).max() / np.abs(ref).max() <= 1e-5
+
+
+class _OracleObstacle(O.Flow):
+    """the oracle flow with lt.Obstacle's units and lettuce's default
+    Obstacle boundaries (equilibrium inlet, anti-bounce-back outlet,
+    fullway bounce-back)"""
+
+    def __init__(self, res, mask, f0, length_x):
+        units = O.UnitConversion(reynolds_number=100, mach_number=0.1,
+                                 characteristic_length_lu=res[0] / length_x,
+                                 characteristic_length_pu=1, characteristic_velocity_pu=1)
+        super().__init__(O.Context("cpu", F64), res, units, O.D2Q9())
+        self.f = torch.as_tensor(f0).to(F64).clone()
+        x = torch.meshgrid(*[units.convert_length_to_pu(torch.arange(n)) for n in res],
+                           indexing="ij")[0]
+        self.post_boundaries = [
+            O.EquilibriumBoundaryPU(self.context, self, torch.abs(x) < 1e-6,
+                                    units.characteristic_velocity_pu * torch.eye(2)[0]),
+            O.AntiBounceBackOutlet([1, 0], self),
+            O.BounceBackBoundary(torch.as_tensor(mask)),
+        ]
+
+
+def test_two_step_2d_default_obstacle_vs_oracle():
+    """lettuce's default Obstacle (anti-bounce-back outlet, fullway
+    bounce-back) at 128 x 64, FP32, 100 steps on the two-step kernel,
+    against the oracle in FP64: 1e-5 relative"""
+    nx, ny = 128, 64
+    mask = _mask(nx, ny)
+    flow = lt.Obstacle(lt.Context("cuda:0", dtype=F32), [nx, ny], 100, 0.1,
+                       domain_length_x=nx / 10.0, stencil=lt.D2Q9())
+    flow.mask = mask
+    flow.initialize()
+    f0 = flow.f.double().cpu().numpy()
+    tau = flow.units.relaxation_parameter_lu
+    sim = lt.Simulation(flow, lt.BGKCollision(tau), [], PRE)
+    assert sim._collide_and_stream.plan(100) == (50, 0)
+    oflow = _OracleObstacle([nx, ny], mask, f0, nx / 10.0)
+    osim = O.Simulation(oflow, O.BGKCollision(tau), [], O.StreamingStrategy.PRE_STREAMING)
+    assert np.array_equal(sim.no_collision_mask.cpu().numpy(), osim.no_collision_mask.numpy())
+    sim(100)
+    osim(100)
+    out = flow.f.double().cpu().numpy()
+    ref = oflow.f.numpy()
+    assert np.abs(out - ref).max() / np.abs(ref).max() <= 1e-5
