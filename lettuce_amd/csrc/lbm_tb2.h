@@ -25,19 +25,21 @@ using TbFn = void (*)(const TbParams);
 TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers, bool wide);
 
 // The two-step kernel for 2D grids with a cell-class map (lbm_tb2d.hip):
-// one block of n2 threads per chunk of cx rows, ring_offset bytes of class
-// table before the LDS ring, tb2d_lds_bytes of dynamic LDS in all.
+// one block of nr x n2 threads (nr = 1 or 2 rows per iteration) per chunk
+// of cx rows, with tb2d_lds_bytes of dynamic LDS (the S1 ring) besides
+// tb2d_static_lds_bytes of its own.
 struct Tb2dParams {
   KParams k;
-  int cx;           // rows (i1) per block
-  int ring_offset;  // byte offset of the S1 ring in the block's LDS
+  int cx;  // rows (i1) per block
 };
 using Tb2dFn = void (*)(const Tb2dParams);
-// kernel for (stencil, storage codec, collision), or nullptr
-Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll);
-// dynamic LDS bytes of a block for nclasses classes, rows of n2 cells and
-// storage_bytes per population; *ring_offset: where the ring starts
-int64_t tb2d_lds_bytes(int nclasses, int n2, int storage_bytes, int64_t* ring_offset);
+// kernel for (stencil, storage codec, collision, nr), or nullptr
+Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll, int nr);
+// dynamic LDS bytes of a block for rows of n2 cells, storage_bytes per
+// population and nr rows per iteration, and the kernel's static LDS bytes
+// (class fields)
+int64_t tb2d_lds_bytes(int n2, int storage_bytes, int nr);
+int64_t tb2d_static_lds_bytes();
 // rows of S1 beyond each end of a block's chunk
 int tb2d_halo();
 

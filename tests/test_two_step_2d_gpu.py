@@ -5,9 +5,9 @@ fullway bounce-back, walls) against single masked steps, to the rounding of
 the collision and boundary arithmetic (the compiler may contract the same
 formulas into FMAs differently in the two kernels); and two-step runs
 against the oracle (lettuce's torch step).  Cases cover a partial last row
-chunk, chunks of one to sixteen rows, rows of 64 to 512 cells, the storage
-formats, and lettuce's default post-streaming order (whose pull steps run
-on the same kernel)."""
+chunk, chunks of one to sixteen rows, rows of 64 to 1024 cells (two rows
+per iteration up to 512, one above), the storage formats, and lettuce's
+default post-streaming order (whose pull steps run on the same kernel)."""
 import numpy as np
 import pytest
 import torch
@@ -31,7 +31,7 @@ COLLISIONS = {"bgk": lambda tau: lt.BGKCollision(tau), "kbc": lambda tau: lt.KBC
 
 def _mask(nx, ny):
     yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
-    return (xx - nx / 4) ** 2 + (yy - ny / 2 + 0.5) ** 2 < (ny / 6) ** 2
+    return (xx - nx / 4) ** 2 + (yy - ny / 2 + 0.5) ** 2 < (min(nx, ny) / 6) ** 2
 
 
 def _channel(storage, nx, ny, outlet, solid, walls=True, dtype=F32):
@@ -72,6 +72,9 @@ CASES = [
     ("f32", 130, 512, "equilibrium", "halfway", "bgk", 0),
     ("f16_shifted", 100, 64, "equilibrium", "halfway", "kbc", 0),
     ("bf16_shifted", 100, 128, "antibounceback", "halfway", "bgk", 5),
+    # rows of more than 512 cells: one row per iteration
+    ("f32", 40, 640, "equilibrium", "halfway", "bgk", 0),
+    ("f16_shifted", 36, 1024, "antibounceback", "fullway", "kbc", 7),
 ]
 
 
