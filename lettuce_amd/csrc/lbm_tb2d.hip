@@ -90,16 +90,6 @@ __device__ __forceinline__ void load_pull_pops(const KParams& p, int cell, const
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// A block-wide barrier that orders LDS only: each wave's LDS operations
-// complete before it, while its global loads stay in flight (a full
-// __syncthreads waits for them too, which would empty the load pipeline
-// every iteration)
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 }  // namespace
 
 template <class S, class ST, int COLL, int NR>
@@ -256,17 +246,17 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
   int k = 0;
   for (; k * NR < 2 * H; ++k) {
     phase_a(k * NR + r);
-    lds_barrier();
+    __syncthreads();
   }
   for (; (k + 1) * NR <= nit; ++k) {
     phase_a(k * NR + r);
-    lds_barrier();
+    __syncthreads();
     phase_b(k * NR + r);
   }
   if (k * NR < nit) {  // a last iteration with fewer than NR rows
     const bool valid = k * NR + r < nit;
     if (valid) phase_a(k * NR + r);
-    lds_barrier();
+    __syncthreads();
     if (valid) phase_b(k * NR + r);
   }
 }
