@@ -78,6 +78,13 @@ __device__ __forceinline__ void load_pull_pops(const KParams& p, int cell, const
   const uint32_t flags = cl.flags[ids.c0], nsm = cl.nsm[ids.c0];
   const bool near = (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
   const uint32_t sq = (uint32_t)p.sq;
+  if (!__any(near || nsm != 0)) {
+    // a wavefront away from boundaries: the plain pull
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q)
+      raw[q] = elem(fin, (uint32_t)q * sq + (uint32_t)nb_in_plane<S>(q, -1, a1, a2));
+    return;
+  }
   // selects, not branches: every lane reads its sources' flags
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) {
@@ -143,6 +150,15 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
     const uint32_t flags = cl.flags[c0], nsm = cl.nsm[c0];
     const bool near = (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
     const int own_pop = prow[1] + cols3[1];  // + q n2: the cell's own population q
+    if (!__any(near || nsm != 0)) {
+      // a wavefront away from boundaries: the plain pull
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int ex = 1 - go<S>(q, 1), ey = 1 - go<S>(q, 2);
+        g[q] = ST::dec(ring[prow[ex] + q * n2 + cols3[ey]], shift_of<ST>(S::w[q], p));
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int ex = 1 - go<S>(q, 1), ey = 1 - go<S>(q, 2);

@@ -238,7 +238,8 @@ def test_channel_2048x512_post_streaming_split_and_forces_at_full_size(knobs):
     """The 2048 x 512 channel with a D = 64 cylinder (halfway bounce-back on
     the cylinder and walls, equilibrium inlet and outlet) at its real size,
     KBC FP32, 200 steps in lettuce's default order: the split post-streaming
-    run equals push steps to the rounding of the collision, no NaN, and the
+    run (its pull steps on the 2D two-step kernel) equals push steps to the
+    rounding of the collision, no NaN, and the
     drag on the cylinder is positive and the same in both runs"""
     nx, ny = 2048, 512
     yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
