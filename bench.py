@@ -63,6 +63,8 @@ WORKLOADS = {
     "tgv2d_d2q9_bgk_2048x512_f32": ("D2Q9", "bgk", "f32", [2048, 512], 200, 0.1),
     # channel + cylinder (D = 64, halfway bounce-back, equilibrium in/outlet)
     "cyl2d_d2q9_kbc_2048x512_f32": ("D2Q9", "kbc", "f32", [2048, 512], 200, 0.05 * 3 ** 0.5),
+    "cyl2d_d2q9_kbc_2048x512_f16s": ("D2Q9", "kbc", "f16_shifted", [2048, 512], 200,
+                                     0.05 * 3 ** 0.5),
     "cyl2d_d2q9_learned_2048x512_f32": ("D2Q9", "learned", "f32", [2048, 512], 200,
                                         0.05 * 3 ** 0.5),
     "cyl2d_d2q9_learned_2048x512_f16s": ("D2Q9", "learned", "f16_shifted", [2048, 512], 200,
