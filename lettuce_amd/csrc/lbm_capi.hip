@@ -392,13 +392,13 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
     // rows per block: the grid's rows over the blocks the device holds at
     // once, at least 8 (each block computes S1 on 4 rows beyond its chunk)
     const int n1 = (int)a->n[1], n2 = (int)a->n[2];
+    // the ring may exceed the default 64 KB of dynamic LDS per block
+    (void)hipFuncSetAttribute((const void*)t2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     tp.cx = knob(K_TB2_PLANES);
     if (tp.cx < 1) {
       tp.cx = 8;
       int per_cu = 0, dev = 0, cus = 0;
-      if (hipFuncSetAttribute((const void*)t2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds) == hipSuccess &&
-          hipGetDevice(&dev) == hipSuccess &&
+      if (hipGetDevice(&dev) == hipSuccess &&
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
           hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)t2, nr * n2,
                                                        (size_t)lds) == hipSuccess &&
@@ -407,7 +407,6 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
         tp.cx = (int)std::max<int64_t>(8, (n1 + resident - 1) / resident);
       }
     }
-    (void)hipFuncSetAttribute((const void*)t2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const unsigned nb = (unsigned)((n1 + tp.cx - 1) / tp.cx);
     for (int64_t k = 0; k < pairs; ++k, ++launch) {
       tp.k.fin = bufs[launch & 1];

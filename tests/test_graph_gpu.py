@@ -46,10 +46,22 @@ def _case(kind):
         flow.initialize()
         coll = lt.LearnedCollision(lt.LearnedParameters.seeded(0, out_scale=0.1))
         return flow, coll, mask
+    if kind == "d2q9_kbc_channel_two_step":
+        # rows of 512 cells: the 2D two-step kernel with 155.6 KB of LDS
+        nx, ny = 48, 512
+        yy, xx = np.meshgrid(np.arange(ny), np.arange(nx))
+        mask = (xx - 12) ** 2 + (yy - 255.5) ** 2 < 6.0 ** 2
+        ctx = lt.Context("cuda:0", dtype=F32)
+        flow = lt.Obstacle(ctx, [nx, ny], 100, 0.1, domain_length_x=nx / 16.0,
+                           stencil=lt.D2Q9(), outlet="equilibrium", solid="halfway", walls=True)
+        flow.mask = mask
+        flow.initialize()
+        return flow, lt.KBCCollision(), mask
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["d3q19_two_step", "d2q9_learned_channel"])
+@pytest.mark.parametrize("kind", ["d3q19_two_step", "d2q9_learned_channel",
+                                  "d2q9_kbc_channel_two_step"])
 def test_captured_steps_replay_like_eager_steps(kind):
     steps = 4  # even: the result stays in the state buffer the graph reads
     flow, coll, mask = _case(kind)
