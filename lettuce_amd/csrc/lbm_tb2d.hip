@@ -21,8 +21,10 @@
 // outlet cell gathers (x - dir, pulled).  Rows wrap in the ring as the grid
 // wraps periodically, and a row is complete in the block, so the periodic
 // wrap along i2 is the ring's own.  The loads of S1 are software-pipelined:
-// the populations of row it + 1 and the class ids of row it + 2 are in
-// flight while the block computes S2 of the current iteration.
+// the populations of row rc + NR and the class ids of row rc + 2 NR are in
+// flight while the block computes S1 and S2 of the current iteration.
+// Wavefronts without boundary cells pull plainly, without reading their
+// sources' classes.
 // S1 values are rounded as a stored step rounds them (stochastic rounding
 // with the first step's bits), so the result equals two step_kernel
 // launches up to the rounding of the collision arithmetic (bit for bit
