@@ -357,23 +357,48 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
     return v;
   };
 
-  // the tile loop is uniform across the wave: every MFMA runs with all lanes
-  // (readfirstlane tells the compiler so: the tile's i0 plane offset then
-  // lives in scalar registers)
-  for (int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
-                                                 (threadIdx.x >> 6));
-       tile < la.ntiles; tile += nwaves) {
-    const int b2 = tile % la.nt2;
+  // D2Q9 pull steps issue the next tile's pull loads before the current
+  // tile's MLP (raw, c0: reloaded in place once the current tile has
+  // decoded them).  The 2048 x 512 channel, a small grid, is bound by
+  // those loads' latency rather than by arithmetic; D3Q27's 27 more
+  // registers would cost it occupancy in an arithmetic-bound kernel.
+  constexpr bool PF = S::Q == 9 && PULL;
+  using X = std::conditional_t<WIDE, long long, int>;
+  // the offsets of a tile: i0 plane, i1 row, and the lane's cell along i2
+  auto coords = [&](int tile, int& i0, int& i1, int& i2, int& b2, X (&a0)[3], int (&a1)[3],
+                    int (&a2)[3]) {
+    b2 = tile % la.nt2;
     const int r = tile / la.nt2;
-    const int i1 = r % p.n1, i0 = r / p.n1;
-    const int i2 = b2 * 64 + lane;
-    using X = std::conditional_t<WIDE, long long, int>;
-    X a0[3];
-    int a1[3], a2[3];
+    i1 = r % p.n1;
+    i0 = r / p.n1;
+    i2 = b2 * 64 + lane;
     axis_offsets(i0, p.n0, (X)p.s0, a0);
     axis_offsets(i1, p.n1, p.s1, a1);
     axis_offsets(i2, p.n2, 1, a2);
-    const int cell = (int)(a0[1] + a1[1] + a2[1]);  // used with boundaries only
+    return (int)(a0[1] + a1[1] + a2[1]);
+  };
+  typename ST::T raw[Q];
+  uint32_t c0 = 0;
+  const int first = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                   (threadIdx.x >> 6));
+  if constexpr (PF) {
+    if (first < la.ntiles) {
+      int j0, j1, j2, c2;
+      X n0_[3];
+      int n1_[3], n2_[3];
+      const int ncell = coords(first, j0, j1, j2, c2, n0_, n1_, n2_);
+      gather_issue<S, ST, MASK>(p, ncell, n0_, n1_, n2_, raw, c0);
+    }
+  }
+
+  // the tile loop is uniform across the wave: every MFMA runs with all lanes
+  // (readfirstlane tells the compiler so: the tile's i0 plane offset then
+  // lives in scalar registers)
+  for (int tile = first; tile < la.ntiles; tile += nwaves) {
+    int i0, i1, i2, b2;
+    X a0[3];
+    int a1[3], a2[3];
+    const int cell = coords(tile, i0, i1, i2, b2, a0, a1, a2);  // used with boundaries only
     using LM = LearnedModel<S>;
     // D3Q27 with a shifted format works on the stored values g = f - w rho0
     // throughout: the update f' = f - Phi(...) is g' = g - Phi(...), and the
@@ -386,7 +411,19 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
     };
     uint32_t nsm, flags, op;
     C f[Q];
-    gather_batched<S, ST, MASK, PULL, RAW, true>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
+    if constexpr (PF) {
+      gather_finish<S, ST, MASK, RAW>(p, cell, a0, a1, a2, ctl, raw, c0, nsm, flags, op, f);
+      const int next = tile + nwaves;
+      if (next < la.ntiles) {
+        int j0, j1, j2, c2;
+        X n0_[3];
+        int n1_[3], n2_[3];
+        const int ncell = coords(next, j0, j1, j2, c2, n0_, n1_, n2_);
+        gather_issue<S, ST, MASK>(p, ncell, n0_, n1_, n2_, raw, c0);
+      }
+    } else {
+      gather_batched<S, ST, MASK, PULL, RAW, true>(p, cell, a0, a1, a2, ctl, nsm, flags, op, f);
+    }
 
     // non-equilibrium part and the MLP inputs
     C feat[6];

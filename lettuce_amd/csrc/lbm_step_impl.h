@@ -291,6 +291,66 @@ __device__ __forceinline__ void gather_batched(const KParams& p, int cell, const
     f[q] = RAW ? ST::val(raw[q]) : ST::dec(raw[q], shift_of<ST>(S::w[q], p));
 }
 
+// The pull gather of gather_batched's SPEC schedule in two halves, for a
+// kernel that issues the next cell's loads before it computes the current
+// one.  gather_issue loads every plain pull source and the cell's own class
+// id into raw and c0; gather_finish applies the class (lanes next to a
+// solid or with no-stream bits load their class ids and own populations, a
+// round later) and decodes into f.
+template <class S, class ST, bool MASK, class X>
+__device__ __forceinline__ void gather_issue(const KParams& p, int cell, const X (&a0)[3],
+                                             const int (&a1)[3], const int (&a2)[3],
+                                             typename ST::T (&raw)[S::Q], uint32_t& c0) {
+  using T = typename ST::T;
+  const T* fin = (const T*)p.fin;
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const X sp = nb_plane<S>(q, -1, a0);
+    raw[q] = elem(at_plane(fin + q * p.sq, sp), in_plane(sp, nb_in_plane<S>(q, -1, a1, a2)));
+  }
+  c0 = MASK ? (uint32_t)p.cls[cell] : 0u;
+}
+
+template <class S, class ST, bool MASK, bool RAW, class X>
+__device__ __forceinline__ void gather_finish(const KParams& p, int cell, const X (&a0)[3],
+                                              const int (&a1)[3], const int (&a2)[3],
+                                              const lbm_cell_class* ctl,
+                                              typename ST::T (&raw)[S::Q], uint32_t c0,
+                                              uint32_t& nsm, uint32_t& flags, uint32_t& op,
+                                              typename ST::C (&f)[S::Q]) {
+  using T = typename ST::T;
+  const T* fin = (const T*)p.fin;
+  nsm = 0;
+  flags = 0;
+  op = LBM_OP_COLLIDE;
+  if constexpr (MASK) {
+    nsm = ctl[c0].no_stream;
+    flags = ctl[c0].flags;
+    op = ctl[c0].op;
+    const bool near =
+        (flags & (LBM_CLASS_NEAR_SOLID | LBM_CLASS_SOLID)) == LBM_CLASS_NEAR_SOLID;
+    if (near || nsm != 0) {
+      uint32_t cid[S::Q];
+      T own[S::Q];
+#pragma unroll
+      for (int q = 1; q < S::Q; ++q) {
+        cid[q] = p.cls[nb_offset<S>(q, -1, a0, a1, a2)];
+        own[q] = fin[q * p.sq + cell];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 1; q < S::Q; ++q) {
+        if (near && (ctl[cid[q]].flags & LBM_CLASS_SOLID)) raw[q] = own[S::opp[q]];
+        else if ((nsm >> q) & 1u) raw[q] = own[q];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q)
+    f[q] = RAW ? ST::val(raw[q]) : ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+}
+
 // the populations of the cell at offset plane + in (its i0 plane, and inside it)
 template <class S, class ST>
 __device__ __forceinline__ void gather_own(const KParams& p, long long plane, int in,
