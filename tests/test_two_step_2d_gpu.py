@@ -213,3 +213,39 @@ def test_two_step_2d_with_stochastic_rounding(knobs):
     diff = np.abs(got - ref)
     assert diff.max() <= 8 * ULPS["bf16_shifted"], diff.max()
     assert (diff > 0).mean() < 0.2 * (other != ref).mean(), ((diff > 0).mean(), (other != ref).mean())
+
+
+@pytest.mark.parametrize("dtype", [F32, torch.float16])
+def test_two_step_2d_equilibrium_boundary_field_vs_oracle(dtype, knobs):
+    """a spatially varying equilibrium boundary (per-cell FP64 field) on the
+    x = 0 row and an interior block of a 24 x 64 D2Q9 grid, BGK, 20 pull
+    steps on the two-step kernel, against the oracle: 1e-5 relative in FP32
+    storage, 5e-3 in shifted FP16 storage (whose stored deviations round to
+    2^-11 relative each step)"""
+    from tests.test_oracle import _Grid, _perturbed
+    from tests.test_parity_gpu import ArrayFlow
+    res = [24, 64]
+    g = torch.Generator().manual_seed(6)
+    velocity = torch.zeros([2, 1, 64], dtype=F64)
+    velocity[0, 0] = 0.05 * torch.sin(torch.linspace(0, np.pi, 64, dtype=F64))
+    pressure = 0.002 * torch.rand([1, 24, 64], generator=g, dtype=F64)
+    mask = torch.zeros(res, dtype=torch.bool)
+    mask[0] = True
+    mask[10:13, 20:30] = True
+    f0 = np.ascontiguousarray(_perturbed(O.D2Q9(), 64, F64, seed=9).f.numpy()[:, :24, :])
+    storage = None if dtype == F32 else "f16_shifted"
+    flow = ArrayFlow(lt.Context("cuda:0", dtype=F32, storage=storage), f0, lt.D2Q9())
+    flow._post = [lt.EquilibriumBoundaryPU(flow.context, flow, mask, velocity, pressure)]
+    sim = lt.Simulation(flow, lt.BGKCollision(0.7), [], PRE)
+    assert sim.cell_classes.bc_field is not None
+    assert sim._collide_and_stream.plan(20) == (10, 0)
+    oflow = _Grid(O.D2Q9(), res, F64, f0)
+    oflow.post_boundaries = [O.EquilibriumBoundaryPU(oflow.context, oflow, mask, velocity,
+                                                     pressure)]
+    osim = O.Simulation(oflow, O.BGKCollision(0.7), [], O.StreamingStrategy.PRE_STREAMING)
+    sim(20)
+    osim(20)
+    got = flow.f.double().cpu().numpy()
+    ref = oflow.f.numpy()
+    tol = 1e-5 if dtype == F32 else 5e-3   # FP16: a few storage ulps per step
+    assert np.abs(got - ref).max() / np.abs(ref).max() <= tol
