@@ -331,7 +331,8 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
     u_rms 0.05, nu 1e-4 in lattice units, seeded field): MLUPS of plain KBC
     with FP32 storage and of the learned collision (seed-0 weights, KBC
     fallback) with shifted FP16 and BF16 storage (BF16 rounded to nearest
-    and stochastically), timed over `timed` steps
+    and stochastically), timed over `timed` steps (with each run's
+    algorithmic fraction of the HBM roofline)
     after `warmup`; and the largest relative kinetic-energy error of each
     against KBC with FP64 storage, every `every` steps over `steps` steps."""
     t_all = time.perf_counter()
@@ -391,7 +392,13 @@ def learned_vs_plain(lt, res=384, steps=1000, every=50, timed=100, warmup=10, u_
             gpu_s, _ = time_gpu(sim, timed, warmup)
             entry["mlups"] = round(timed * res ** 3 / 1e6 / gpu_s, 1)
             pairs, singles = sim._collide_and_stream.plan(timed)
-            entry["kernel"] = "two-step" if pairs and not singles else "single step"
+            two_step = pairs and not singles
+            entry["kernel"] = "two-step" if two_step else "single step"
+            # algorithmic bytes of one memory pass (one read and one write of
+            # every population) per second, as a fraction of HBM peak; a
+            # two-step launch makes two updates per pass
+            passes = entry["mlups"] * 1e6 / (2 if two_step else 1)
+            entry["roofline_frac"] = round(2 * 27 * BYTES[storage] * passes / (PEAK_HBM_GBS * 1e9), 4)
         out[name] = entry
         del sim, flow, energy
         torch.cuda.empty_cache()
