@@ -219,22 +219,24 @@ static TbFn two_step_kernel(const lbm_step_args* a) {
                          wide_offsets(a->storage, a->n, a->stride));
 }
 
-// the two-step kernel for a 2D grid with boundaries, if one applies: D2Q9
-// with a cell-class map, pull streaming, a collision other than the learned
-// one, 32- or 16-bit storage, contiguous rows of a multiple of 64 and at
-// most 1024 cells, element offsets within 32 bits (bytes), and the LDS
-// ring within the device's per-block limit.  Two rows per iteration where
+// the two-step kernel for a 2D grid, if one applies: D2Q9 (with or without
+// a cell-class map), pull streaming, a collision other than the learned
+// one, 32- or 16-bit storage, rows of a multiple of 64 and at most 1024
+// cells (contiguous, where a class map indexes them), element offsets
+// within 32 bits (bytes), and the LDS ring within the device's per-block
+// limit.  Two rows per iteration where
 // a block of 2 n2 threads and its ring fit (measured on the 2048 x 512
 // channel, BGK FP32: 61.2k MLUPS with one row, 77.2k with two), else one.
 // *lds: the block's dynamic LDS, *nr: rows per iteration.
 static Tb2dFn two_step_2d(const lbm_step_args* a, int64_t* lds, int* nr) {
   if (knob(K_TWO_STEP) == 0) return nullptr;
-  if (a->stencil != LBM_D2Q9 || !a->cell_class || a->streaming != LBM_PRE_STREAMING) return nullptr;
+  if (a->stencil != LBM_D2Q9 || a->streaming != LBM_PRE_STREAMING) return nullptr;
   if (a->collision == LBM_COLL_LEARNED || a->storage == LBM_F64) return nullptr;
   if (a->n[0] != 1 || a->n[2] % 64 != 0 || a->n[2] > 1024) return nullptr;
-  if (a->stride[3] != 1 || a->stride[2] != a->n[2]) return nullptr;
+  if (a->stride[3] != 1 || (a->cell_class && a->stride[2] != a->n[2])) return nullptr;
   const int bytes = storage_bytes(a->storage);
-  if ((a->stride[0] * 8 + a->n[1] * a->n[2]) * bytes >= ((int64_t)1 << 32)) return nullptr;
+  const int64_t last = a->stride[0] * 8 + a->stride[2] * (a->n[1] - 1) + a->n[2] - 1;
+  if ((last + 1) * bytes >= ((int64_t)1 << 32)) return nullptr;
   int dev = 0, max_lds = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
@@ -243,7 +245,8 @@ static Tb2dFn two_step_2d(const lbm_step_args* a, int64_t* lds, int* nr) {
     *nr = rows;
     *lds = tb2d_lds_bytes((int)a->n[2], bytes, rows);
     if (rows * a->n[2] <= 1024 && *lds + tb2d_static_lds_bytes() <= max_lds)
-      return pick_tb2d_kernel(a->stencil, codec_id(a->storage, a->sr_seed), a->collision, rows);
+      return pick_tb2d_kernel(a->stencil, codec_id(a->storage, a->sr_seed), a->collision, rows,
+                              a->cell_class != nullptr);
   }
   return nullptr;
 }

@@ -24,7 +24,8 @@ using TbFn = void (*)(const TbParams);
 // not fit 32 bits (one barrier per plane)
 TbFn pick_tb2_kernel(int stencil, int storage, int coll, int barriers, bool wide);
 
-// The two-step kernel for 2D grids with a cell-class map (lbm_tb2d.hip):
+// The two-step kernel for 2D grids, with or without a cell-class map
+// (lbm_tb2d.hip):
 // one block of nr x n2 threads (nr = 1 or 2 rows per iteration) per chunk
 // of cx rows, with tb2d_lds_bytes of dynamic LDS (the S1 ring) besides
 // tb2d_static_lds_bytes of its own.
@@ -33,8 +34,9 @@ struct Tb2dParams {
   int cx;  // rows (i1) per block
 };
 using Tb2dFn = void (*)(const Tb2dParams);
-// kernel for (stencil, storage codec, collision, nr), or nullptr
-Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll, int nr);
+// kernel for (stencil, storage codec, collision, nr, with a class map), or
+// nullptr
+Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll, int nr, bool mask);
 // dynamic LDS bytes of a block for rows of n2 cells, storage_bytes per
 // population and nr rows per iteration, and the kernel's static LDS bytes
 // (class fields)

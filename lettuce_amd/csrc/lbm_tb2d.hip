@@ -1,5 +1,5 @@
-// Two fused pull steps per pass over memory for 2D lattices with boundaries
-// (D2Q9 with a cell-class map): f_out = K(K(f_in)) with K the masked pull
+// Two fused pull steps per pass over memory for 2D lattices (D2Q9), with a
+// cell-class map or periodic without one: f_out = K(K(f_in)) with K the pull
 // step of step_kernel (halfway bounce-back, no-stream bits, fullway
 // bounce-back, equilibrium boundaries, equilibrium and anti-bounce-back
 // outlets).  The intermediate populations stay in LDS, so a pair of steps
@@ -50,9 +50,16 @@ struct PullIds {
   uint32_t src[S::Q];
 };
 
-template <class S>
+// (a grid without a class map: every cell is class 0, a plain collision)
+template <class S, bool MASK>
 __device__ __forceinline__ void load_pull_ids(const KParams& p, int cell, const int (&a1)[3],
                                               const int (&a2)[3], PullIds<S>& ids) {
+  if constexpr (!MASK) {
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) ids.src[q] = 0;
+    ids.c0 = 0;
+    return;
+  }
   ids.c0 = elem(p.cls, (uint32_t)cell);
 #pragma unroll
   for (int q = 1; q < S::Q; ++q) ids.src[q] = elem(p.cls, (uint32_t)nb_in_plane<S>(q, -1, a1, a2));
@@ -101,7 +108,7 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 
 }  // namespace
 
-template <class S, class ST, int COLL, int NR>
+template <class S, class ST, int COLL, int NR, bool MASK>
 __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
   using C = typename ST::C;
   using T = typename ST::T;
@@ -114,10 +121,16 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
   T* ring = (T*)tb2d_lds;                          // [R][Q][n2]
   uint8_t* cring = (uint8_t*)(ring + R * Q * n2);  // [R][n2] class ids
   uint8_t* sring = cring + R * n2;                 // [R][n2] SOLID flags
-  for (int i = threadIdx.x; i < p.nclasses; i += blockDim.x) {
-    cl.flags[i] = p.ct[i].flags;
-    cl.op[i] = p.ct[i].op;
-    cl.nsm[i] = p.ct[i].no_stream;
+  if (MASK) {
+    for (int i = threadIdx.x; i < p.nclasses; i += blockDim.x) {
+      cl.flags[i] = p.ct[i].flags;
+      cl.op[i] = p.ct[i].op;
+      cl.nsm[i] = p.ct[i].no_stream;
+    }
+  } else if (threadIdx.x == 0) {
+    cl.flags[0] = 0;
+    cl.op[0] = LBM_OP_COLLIDE;
+    cl.nsm[0] = 0;
   }
 
   // NR rows per iteration: thread row r (whole wavefronts, n2 % 64 == 0)
@@ -195,12 +208,12 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
   {
     int a1[3];
     const int cell = rows(r, a1);
-    load_pull_ids<S>(p, cell, a1, a2, ids);
+    load_pull_ids<S, MASK>(p, cell, a1, a2, ids);
     load_pull_pops<S, ST>(p, cell, a1, a2, ids, cl, raw);
     c0 = ids.c0;
     if (r + NR < nit) {
       int b1[3];
-      load_pull_ids<S>(p, rows(r + NR, b1), b1, a2, ids);
+      load_pull_ids<S, MASK>(p, rows(r + NR, b1), b1, a2, ids);
     }
   }
 
@@ -220,7 +233,7 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
       c0 = ids.c0;
       if (rc + 2 * NR < nit) {
         int c1[3];
-        load_pull_ids<S>(p, rows(rc + 2 * NR, c1), c1, a2, ids);
+        load_pull_ids<S, MASK>(p, rows(rc + 2 * NR, c1), c1, a2, ids);
       }
     }
     if (op == LBM_OP_COLLIDE) collide<S, C, COLL>(f, cp);
@@ -282,31 +295,32 @@ __global__ void __launch_bounds__(1024) tb2d_kernel(const Tb2dParams P) {
 namespace {
 
 template <class S, class ST, int COLL>
-Tb2dFn pick_tb2d_rows(int nr) {
-  return nr == 2 ? tb2d_kernel<S, ST, COLL, 2> : tb2d_kernel<S, ST, COLL, 1>;
+Tb2dFn pick_tb2d_rows(int nr, bool mask) {
+  if (mask) return nr == 2 ? tb2d_kernel<S, ST, COLL, 2, true> : tb2d_kernel<S, ST, COLL, 1, true>;
+  return nr == 2 ? tb2d_kernel<S, ST, COLL, 2, false> : tb2d_kernel<S, ST, COLL, 1, false>;
 }
 
 template <class S, class ST>
-Tb2dFn pick_tb2d_coll(int coll, int nr) {
+Tb2dFn pick_tb2d_coll(int coll, int nr, bool mask) {
   switch (coll) {
-    case LBM_COLL_NONE: return pick_tb2d_rows<S, ST, LBM_COLL_NONE>(nr);
-    case LBM_COLL_BGK: return pick_tb2d_rows<S, ST, LBM_COLL_BGK>(nr);
-    case LBM_COLL_TRT: return pick_tb2d_rows<S, ST, LBM_COLL_TRT>(nr);
-    case LBM_COLL_KBC: return pick_tb2d_rows<S, ST, LBM_COLL_KBC>(nr);
-    case LBM_COLL_BGK_GUO: return pick_tb2d_rows<S, ST, LBM_COLL_BGK_GUO>(nr);
+    case LBM_COLL_NONE: return pick_tb2d_rows<S, ST, LBM_COLL_NONE>(nr, mask);
+    case LBM_COLL_BGK: return pick_tb2d_rows<S, ST, LBM_COLL_BGK>(nr, mask);
+    case LBM_COLL_TRT: return pick_tb2d_rows<S, ST, LBM_COLL_TRT>(nr, mask);
+    case LBM_COLL_KBC: return pick_tb2d_rows<S, ST, LBM_COLL_KBC>(nr, mask);
+    case LBM_COLL_BGK_GUO: return pick_tb2d_rows<S, ST, LBM_COLL_BGK_GUO>(nr, mask);
   }
   return nullptr;
 }
 
 }  // namespace
 
-Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll, int nr) {
+Tb2dFn pick_tb2d_kernel(int stencil, int codec, int coll, int nr, bool mask) {
   if (stencil != LBM_D2Q9 || (nr != 1 && nr != 2)) return nullptr;
   switch (codec) {
-    case LBM_F32: return pick_tb2d_coll<D2Q9, StF32>(coll, nr);
-    case LBM_F16_SHIFTED: return pick_tb2d_coll<D2Q9, StF16>(coll, nr);
-    case LBM_BF16_SHIFTED: return pick_tb2d_coll<D2Q9, StBF16>(coll, nr);
-    case kCodecBF16SR: return pick_tb2d_coll<D2Q9, StBF16SR>(coll, nr);
+    case LBM_F32: return pick_tb2d_coll<D2Q9, StF32>(coll, nr, mask);
+    case LBM_F16_SHIFTED: return pick_tb2d_coll<D2Q9, StF16>(coll, nr, mask);
+    case LBM_BF16_SHIFTED: return pick_tb2d_coll<D2Q9, StBF16>(coll, nr, mask);
+    case kCodecBF16SR: return pick_tb2d_coll<D2Q9, StBF16SR>(coll, nr, mask);
   }
   return nullptr;
 }

@@ -249,3 +249,48 @@ def test_two_step_2d_equilibrium_boundary_field_vs_oracle(dtype, knobs):
     ref = oflow.f.numpy()
     tol = 1e-5 if dtype == F32 else 5e-3   # FP16: a few storage ulps per step
     assert np.abs(got - ref).max() / np.abs(ref).max() <= tol
+
+
+def _periodic_f0(nx, ny, seed=4):
+    """a perturbed equilibrium of a random velocity field on a periodic
+    nx x ny D2Q9 grid"""
+    from tests.test_oracle import _Grid
+    g = torch.Generator().manual_seed(seed)
+    oflow = _Grid(O.D2Q9(), [nx, ny], F64)
+    u = 0.05 * (torch.rand([2, nx, ny], generator=g, dtype=F64) - 0.5)
+    f0 = oflow.equilibrium(oflow, rho=torch.ones([1, nx, ny], dtype=F64), u=u)
+    return (f0 * (1 + 0.02 * torch.rand(f0.shape, generator=g, dtype=F64))).numpy()
+
+
+@pytest.mark.parametrize("storage,layout,nx,ny,coll", [
+    ("f32", "soa", 40, 64, "bgk"), ("f32", "rows", 37, 128, "kbc"),
+    ("f32", "soa", 24, 640, "trt"), ("f16_shifted", "soa", 33, 512, "bgk"),
+    ("bf16_shifted", "rows", 20, 256, "none"), ("f32", "soa", 16, 128, "none")])
+def test_two_step_2d_periodic_matches_single_steps(storage, layout, nx, ny, coll, knobs):
+    """periodic D2Q9 grids without a class map run the same kernel with
+    every cell a plain collision: bit for bit without collision, otherwise
+    to the rounding of the collision arithmetic, in both memory layouts"""
+    from tests.test_parity_gpu import ArrayFlow
+    f0 = _periodic_f0(nx, ny)
+
+    def run(two_step, steps):
+        knobs(two_step=1 if two_step else 0)
+        ctx = lt.Context("cuda:0", dtype=F32, storage=storage, layout=layout)
+        flow = ArrayFlow(ctx, f0, lt.D2Q9())
+        sim = lt.Simulation(flow, COLLISIONS[coll](0.6), [], PRE)
+        plan = sim._collide_and_stream.plan(steps)
+        sim(steps)
+        torch.cuda.synchronize()
+        bits = flow._state.detach().cpu().contiguous().view(-1).view(torch.uint8).numpy()
+        return bits, flow.f.double().cpu().numpy(), plan
+    for steps in (4, 7):
+        rb, ref, p0 = run(False, steps)
+        gb, got, p1 = run(True, steps)
+        assert p0 == (0, steps)
+        pairs = steps // 2 - (steps // 2) % 2
+        assert p1 == (pairs, steps - 2 * pairs)
+        if coll == "none":
+            assert np.array_equal(gb, rb), steps
+        else:
+            tol = steps * ULPS[storage] * (50 if coll == "kbc" else 1)
+            assert np.abs(got - ref).max() <= tol, (steps, np.abs(got - ref).max())
