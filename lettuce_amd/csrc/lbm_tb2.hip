@@ -123,36 +123,61 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
 #pragma unroll
   for (int q = 0; q < S::Q; ++q) h1[q] = h2[q] = C(0);
 
+  using X = std::conditional_t<WIDE, long long, int>;
+  // D3Q27 (one block per CU at up to 168 VGPRs) issues plane it + 1's loads
+  // right after plane it's decode, into the same registers, so they stay in
+  // flight over S1, the barrier and phase B.  Measured on 256^3 KBC FP32:
+  // 37.1k -> 39.5k MLUPS.  D3Q19 keeps the loads in phase A: 19 more VGPRs
+  // leave one block per CU instead of two (FP32 63.9k -> 62.4k, FP16 106k ->
+  // 73k), or spill at two (FP32 50.9k).
+  constexpr bool PF = S::Q == 27 && !WIDE;
+  T raw[S::Q];
+  auto issue = [&](int itl) {
+    int xa = x0 - 1 + itl;
+    xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
+    X ax[3];
+    axis_offsets(xa, p.n0, (X)p.s0, ax);
+#pragma unroll
+    for (int q = 0; q < S::Q; ++q) {
+      const int in = ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)];
+      if constexpr (WIDE)
+        raw[q] = elem(scalar_base(fin + q * p.sq + ax[1 - go<S>(q, 0)]), in);
+      else
+        raw[q] = elem(fin + q * p.sq, ax[1 - go<S>(q, 0)] + in);
+    }
+  };
+  // the last iteration's prefetch reads plane x0 + cx + 1 (wrapped into the
+  // lattice) and leaves it unused
+  if (PF && active) issue(0);
+
   for (int it = 0; it < cx + 2; ++it) {
     // ---- A: S1 of plane xa = x0 - 1 + it (ring counter it) ----
     if (active) {
       int xa = x0 - 1 + it;
       xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
-      using X = std::conditional_t<WIDE, long long, int>;
       X ax[3];
       axis_offsets(xa, p.n0, (X)p.s0, ax);
-      // The plane's loads go out together, ahead of their decodes, and (for
-      // 16-bit formats) at raised wave priority, so that a wave reaching
-      // this point issues them before other waves' arithmetic.  Measured on
-      // D3Q19 BGK 256^3: FP16 89.7k -> 94.4k MLUPS from the batching and
-      // -> 98.3k with the priority; FP32 59.6k -> 59.8k from the batching
-      // but 58.2k with the priority, so FP32 runs without it.
+      // Without PF the plane's loads go out together, ahead of their
+      // decodes, and (for 16-bit formats) at raised wave priority, so that a
+      // wave reaching this point issues them before other waves' arithmetic.
+      // Measured on D3Q19 BGK 256^3: FP16 89.7k -> 94.4k MLUPS from the
+      // batching and -> 98.3k with the priority; FP32 59.6k -> 59.8k from
+      // the batching but 58.2k with the priority, so FP32 runs without it.
       constexpr bool PRIO = PRIORITIES && sizeof(T) == 2;
-      if (PRIO) __builtin_amdgcn_s_setprio(3);
-      __builtin_amdgcn_sched_barrier(0);
-      T raw[S::Q];
-#pragma unroll
-      for (int q = 0; q < S::Q; ++q) {
-        const int in = ay[1 - go<S>(q, 1)] + az[1 - go<S>(q, 2)];
-        if constexpr (WIDE)
-          raw[q] = elem(scalar_base(fin + q * p.sq + ax[1 - go<S>(q, 0)]), in);
-        else
-          raw[q] = elem(fin + q * p.sq, ax[1 - go<S>(q, 0)] + in);
+      if constexpr (!PF) {
+        if (PRIO) __builtin_amdgcn_s_setprio(3);
+        __builtin_amdgcn_sched_barrier(0);
+        issue(it);
+        __builtin_amdgcn_sched_barrier(0);
+        if (PRIO) base_prio();
       }
-      __builtin_amdgcn_sched_barrier(0);
-      if (PRIO) base_prio();
 #pragma unroll
       for (int q = 0; q < S::Q; ++q) cur[q] = ST::dec(raw[q], shift_of<ST>(S::w[q], p));
+      if constexpr (PF) {
+        __builtin_amdgcn_sched_barrier(0);
+        issue(it + 1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       collide<S, C, COLL>(cur, cp);
       // S1 rounded to the storage format as a single step would store it,
       // stochastic rounding keyed to the first step and this cell
