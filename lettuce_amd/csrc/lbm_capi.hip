@@ -190,16 +190,16 @@ static int block_size_for(int64_t n2) {
 // D3Q27 without boundaries, pull streaming, z a multiple of 64 and y of
 // TB_Y.  Used by default for D3Q19 in every format and for D3Q27 in FP32,
 // with one barrier per plane.  Measured on MI355X, TGV 256^3: D3Q19 BGK FP32
-// 38.3k (single steps) -> 63.6k MLUPS, FP16-shifted 100k; D3Q27 KBC FP32
-// 26.5k -> 29.4k, but D3Q27 KBC BF16 46.4k -> 42.8k (its arithmetic, not
+// 38.3k (single steps) -> 63.4k MLUPS, FP16-shifted 106k; D3Q27 KBC FP32
+// 26.7k -> 40.0k, but D3Q27 KBC BF16 45.9k -> 39.7k (its arithmetic, not
 // its memory traffic, bounds it, and the two-step kernel recomputes the
 // halo rows), so 16-bit D3Q27 stays on single steps.
 // The knob two_step = 0 turns it off, = 2 uses it wherever it applies;
 // two_step_planes sets the x chunk, two_step_barriers = 1 or 2 the variant
-// with one barrier per plane or two with shallower LDS rings.  A variant
-// that loaded the next plane during phase B measured slower (53.0k vs 57.8k
-// MLUPS: its 116 VGPRs left one block per CU) and was removed, and so was
-// the plain tile order (XCD order: 54.7k -> 57.8k MLUPS).
+// with one barrier per plane or two with shallower LDS rings.  For D3Q19,
+// loading the next plane during the current one measured slower (its extra
+// VGPRs leave one block per CU; D3Q27 has one anyway and does load ahead),
+// and so did the plain tile order (XCD order: 54.7k -> 57.8k MLUPS).
 static TbFn two_step_kernel(const lbm_step_args* a) {
   const int mode = knob(K_TWO_STEP);
   if (mode == 0) return nullptr;

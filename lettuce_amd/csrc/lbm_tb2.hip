@@ -133,6 +133,9 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
   constexpr bool PF = S::Q == 27 && !WIDE;
   T raw[S::Q];
   auto issue = [&](int itl) {
+    // the prefetch after the chunk's last plane reloads that plane (unused):
+    // one plane further could lie past the array where n0 == 1
+    if constexpr (PF) itl = min(itl, cx + 1);
     int xa = x0 - 1 + itl;
     xa = xa < 0 ? xa + p.n0 : (xa >= p.n0 ? xa - p.n0 : xa);
     X ax[3];
@@ -146,8 +149,6 @@ __global__ void __launch_bounds__(TB_THREADS) tb2_kernel(const TbParams P) {
         raw[q] = elem(fin + q * p.sq, ax[1 - go<S>(q, 0)] + in);
     }
   };
-  // the last iteration's prefetch reads plane x0 + cx + 1 (wrapped into the
-  // lattice) and leaves it unused
   if (PF && active) issue(0);
 
   for (int it = 0; it < cx + 2; ++it) {

@@ -3,7 +3,7 @@ bit without collision (the streaming, the LDS rings and the z-edge cells),
 and to the rounding of the collision arithmetic with BGK and TRT (the
 compiler may contract the same formulas into FMAs differently in the two
 kernels).  Cases cover several z and y tiles, a partial last x chunk, a
-single y tile that wraps onto itself, both memory layouts, the three storage
+single y tile that wraps onto itself, grids of one to three x planes, both memory layouts, the three storage
 formats, odd step counts and an odd number of pairs (which runs one pair as
 two single steps); and the kernel against the oracle's BGK."""
 import numpy as np
@@ -159,6 +159,27 @@ def test_two_step_kernel_d3q27(coll, storage, layout, knobs):
         # collision moves it further than in BGK
         tol = steps * ULPS[storage] * (50 if coll == "kbc" else 1)
         assert err <= tol, (steps, err)
+
+
+@pytest.mark.parametrize("n0", [1, 2, 3])
+@pytest.mark.parametrize("stencil,coll", [(O.D3Q19, "none"), (O.D3Q27, "none"), (O.D3Q27, "kbc")])
+def test_two_step_kernel_on_thin_x_extents(stencil, coll, n0, knobs):
+    """Grids of one to three x planes, one chunk each: the x neighbours wrap
+    onto the chunk itself, and D3Q27's load of the plane after the chunk
+    stays inside the array"""
+    f0 = _f0([n0, 8, 64], stencil=stencil)
+    make = {"kbc": lambda: lt.KBCCollision(), "none": lambda: lt.NoCollision()}[coll]
+    for steps in (4, 5):
+        knobs(two_step="0")
+        ref_bits, ref = _run(f0, "f32", "soa", make(), steps, pairs=0)
+        knobs(two_step="2")
+        pairs = steps // 2 - (steps // 2) % 2
+        got_bits, got = _run(f0, "f32", "soa", make(), steps, pairs=pairs)
+        if coll == "none":
+            assert np.array_equal(got_bits, ref_bits), (steps, int((got_bits != ref_bits).sum()))
+        else:
+            err = np.abs(got - ref).max()
+            assert err <= steps * ULPS["f32"] * 50, (steps, err)
 
 
 @pytest.mark.parametrize("stencil,coll,storage,layout", [
