@@ -360,8 +360,9 @@ __global__ void __launch_bounds__(256, 3) learned_kernel(const KParams p, const 
   // D2Q9 pull steps issue the next tile's pull loads before the current
   // tile's MLP (raw, c0: reloaded in place once the current tile has
   // decoded them).  The 2048 x 512 channel, a small grid, is bound by
-  // those loads' latency rather than by arithmetic; D3Q27's 27 more
-  // registers would cost it occupancy in an arithmetic-bound kernel.
+  // those loads' latency rather than by arithmetic.  D3Q27 with the same
+  // prefetch measured slower (learned BF16 256^3: 33.3k against 37.3k MLUPS
+  // at two waves per SIMD; spills at three).
   constexpr bool PF = S::Q == 9 && PULL;
   using X = std::conditional_t<WIDE, long long, int>;
   // the offsets of a tile: i0 plane, i1 row, and the lane's cell along i2
