@@ -425,7 +425,7 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
     tp.nyt = (int)(a->n[1] / TB_Y);
     // x chunk: 64 planes, or 128 where 64-plane tiles would make between one
     // and two rounds of the blocks the device holds at once, so that they
-    // make one.  Measured on MI355X: D3Q19 256^3 (512 tiles, 512 resident
+    // make one (or 48, below).  Measured on MI355X: D3Q19 256^3 (512 tiles, 512 resident
     // blocks) and 1024^3 (32768 tiles) are fastest at 64 planes (larger
     // chunks: -4 % to -17 %); D3Q27 KBC FP32 256^3, one block per CU, runs
     // 30.7k MLUPS at 64 planes and 31.3k at 128.
@@ -440,7 +440,17 @@ static void run_pull_steps(const lbm_step_args* a, const KParams& p, KernelFn fn
           per_cu > 0) {
         const int64_t resident = (int64_t)per_cu * cus;
         const int64_t tiles64 = (int64_t)tp.nzt * tp.nyt * ((a->n[0] + 63) / 64);
+        // one block per CU (D3Q27): otherwise 48 planes where the rounds of
+        // resident blocks times the planes a block sweeps (cx + 2) come out
+        // lower than for 64.  D3Q27 KBC FP32 384^3: 64 planes make 6.75
+        // rounds, 48 make 9; 35.6k -> 36.2k-37.5k MLUPS (the model ranks the
+        // seven chunk lengths measured there, 32 to 192, in measured order).
+        auto cost = [&](int64_t cx) {
+          const int64_t tiles = (int64_t)tp.nzt * tp.nyt * ((a->n[0] + cx - 1) / cx);
+          return (tiles + resident - 1) / resident * (cx + 2);
+        };
         if (tiles64 > resident && tiles64 <= 2 * resident) tp.cx = 128;
+        else if (per_cu == 1 && cost(48) < cost(64)) tp.cx = 48;
       }
     }
     const int64_t nxt = (a->n[0] + tp.cx - 1) / tp.cx;
